@@ -1,0 +1,202 @@
+"""Python binding of the C ABI in include/chx.h (ctypes over libchx.so).
+
+This mirrors the reference's operator interface for the correlator path --
+the OpenCL program's three kernels `offsetAccumulateElements`, `preseed` and
+`corr` (offset_accumulator.cl:7, preseed_multifreq_UT.cl:19,
+pairwise_correlator_UT.cl:22) and the host loop that chains them
+(main_wrapper.c:691-778) -- with the same argument meaning: packed 4-bit
+input [time][freq][element], block-order int32 output, the
+upper_triangle_convention flag and time_accum.
+
+There is no CPU fallback: if libchx.so is missing, importing raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libchx.so")
+HEADER = os.path.join(REPO, "include", "chx.h")
+
+CONFERENCE, HIGHLY_PACKED, INT8_MFMA = 0, 1, 2
+METHOD_NAMES = {CONFERENCE: "conference", HIGHLY_PACKED: "highly_packed", INT8_MFMA: "int8_mfma"}
+
+SUCCESS = 0
+ERR_INVALID_ARGUMENT, ERR_SHAPE, ERR_TIME_ACCUM, ERR_NO_DEVICE = 1, 2, 3, 4
+ERR_OUT_OF_MEMORY, ERR_HIP, ERR_BUDGET, ERR_BUSY, ERR_NOT_FOUND = 5, 6, 7, 8, 9
+
+
+class ChxError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        self.status = status
+        detail = lib().chx_last_error_detail().decode()
+        msg = f"{where}: {lib().chx_strerror(status).decode()} (status {status})"
+        if detail and status == ERR_HIP:
+            msg += f" [{detail}]"
+        super().__init__(msg)
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("num_elements", ctypes.c_int32),
+        ("num_freq", ctypes.c_int32),
+        ("time_steps", ctypes.c_int64),
+        ("time_accum", ctypes.c_int32),
+        ("upper_triangle_convention", ctypes.c_int32),
+        ("kernel_batch", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+    ]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built: run `make -C chime-x_amd` "
+                          "(or __graft_entry__.build()); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+    sz = ctypes.POINTER(ctypes.c_size_t)
+    L.chx_plan_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(vp)]
+    L.chx_plan_destroy.argtypes = [vp]
+    L.chx_plan_get_config.argtypes = [vp, ctypes.POINTER(Config)]
+    L.chx_plan_sizes.argtypes = [vp, sz, sz, sz]
+    for name in ("chx_offset_accumulate", "chx_preseed", "chx_corr", "chx_correlate"):
+        getattr(L, name).argtypes = [vp, vp, vp, vp]
+    L.chx_correlate_host.argtypes = [vp, vp, vp]
+    L.chx_num_blocks.argtypes = [i32]
+    L.chx_num_blocks.restype = i64
+    L.chx_block_maps.argtypes = [i32, vp, vp]
+    L.chx_blocks_to_triangle.argtypes = [i32, i32, vp, vp]
+    L.chx_blocks_to_triangle_device.argtypes = [i32, i32, vp, vp, vp]
+    L.chx_strerror.argtypes = [ctypes.c_int]
+    L.chx_strerror.restype = ctypes.c_char_p
+    L.chx_last_error_detail.restype = ctypes.c_char_p
+    L.chx_max_time_accum.argtypes = [i32]
+    L.chx_max_time_accum.restype = i32
+    L.chx_version.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def header_symbols() -> list[str]:
+    """Every function the public header declares."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(chx_[a-z0-9_]+)\s*\(", text)))
+
+
+def _check(rc: int, where: str) -> None:
+    if rc != SUCCESS:
+        raise ChxError(rc, where)
+
+
+def _p(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def num_blocks(N: int) -> int:
+    return int(lib().chx_num_blocks(N))
+
+
+def block_maps(N: int):
+    nb = num_blocks(N)
+    x = np.empty(nb, np.uint32)
+    y = np.empty(nb, np.uint32)
+    _check(lib().chx_block_maps(N, _p(x), _p(y)), "chx_block_maps")
+    return x, y
+
+
+def blocks_to_triangle(blocks: np.ndarray, N: int) -> np.ndarray:
+    blocks = np.ascontiguousarray(blocks, np.int32)
+    F = blocks.size // (num_blocks(N) * 2048)
+    tri = np.empty((F, N * (N + 1) // 2, 2), np.int32)
+    _check(lib().chx_blocks_to_triangle(F, N, _p(blocks), _p(tri)), "chx_blocks_to_triangle")
+    return tri
+
+
+def device_count() -> int:
+    return int(lib().chx_device_count())
+
+
+class Plan:
+    """One correlator configuration on one device (chx_plan)."""
+
+    def __init__(self, num_elements: int, num_freq: int, time_steps: int,
+                 time_accum: int = 256, upper_triangle_convention: int = 1,
+                 kernel_batch: int = INT8_MFMA, device: int = 0):
+        self.cfg = Config(num_elements, num_freq, time_steps, time_accum,
+                          upper_triangle_convention, kernel_batch, device)
+        h = ctypes.c_void_p()
+        _check(lib().chx_plan_create(ctypes.byref(self.cfg), ctypes.byref(h)),
+               "chx_plan_create")
+        self._h = h
+        ib, ob, sb = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+        _check(lib().chx_plan_sizes(h, ctypes.byref(ib), ctypes.byref(ob), ctypes.byref(sb)),
+               "chx_plan_sizes")
+        self.input_bytes, self.output_bytes, self.sums_bytes = ib.value, ob.value, sb.value
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    @property
+    def output_shape(self):
+        return (self.cfg.num_freq, num_blocks(self.cfg.num_elements), 32, 32, 2)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().chx_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- host pointers ------------------------------------------------------
+    def correlate(self, data: np.ndarray) -> np.ndarray:
+        """[T][F][N] uint8 -> block-order int32 [F][NB][32][32][2]."""
+        data = np.ascontiguousarray(data, np.uint8)
+        if data.nbytes != self.input_bytes:
+            raise ValueError(f"input has {data.nbytes} bytes, plan expects {self.input_bytes}")
+        out = np.empty(self.output_shape, np.int32)
+        _check(lib().chx_correlate_host(self._h, _p(data), _p(out)), "chx_correlate_host")
+        return out
+
+    # -- device pointers (ints: e.g. torch tensor .data_ptr()) -------------
+    def correlate_device(self, d_input: int, d_vis: int, stream: int = 0) -> None:
+        _check(lib().chx_correlate(self._h, d_input, d_vis, stream or None), "chx_correlate")
+
+    def offset_accumulate(self, d_input: int, d_sums: int, stream: int = 0) -> None:
+        _check(lib().chx_offset_accumulate(self._h, d_input, d_sums, stream or None),
+               "chx_offset_accumulate")
+
+    def preseed(self, d_sums: int, d_vis: int, stream: int = 0) -> None:
+        _check(lib().chx_preseed(self._h, d_sums, d_vis, stream or None), "chx_preseed")
+
+    def corr(self, d_input: int, d_vis: int, stream: int = 0) -> None:
+        _check(lib().chx_corr(self._h, d_input, d_vis, stream or None), "chx_corr")
+
+
+def blocks_to_triangle_device(num_freq: int, N: int, d_blocks: int, d_tri: int,
+                              stream: int = 0) -> None:
+    _check(lib().chx_blocks_to_triangle_device(num_freq, N, d_blocks, d_tri, stream or None),
+           "chx_blocks_to_triangle_device")

@@ -1,0 +1,247 @@
+// chx_api.hip -- the C ABI (include/chx.h): plans, validation, dispatch and
+// the host-side layout helpers.  Replaces the reference host runtime
+// (main_wrapper.c:238-572 set-up, :691-778 per-iteration kernel order).
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "chx_internal.h"
+
+namespace chx {
+
+static thread_local char g_last_error[256];
+
+int hip_status(hipError_t e) {
+  snprintf(g_last_error, sizeof(g_last_error), "%s: %s", hipGetErrorName(e),
+           hipGetErrorString(e));
+  if (e == hipErrorOutOfMemory) return CHX_ERR_OUT_OF_MEMORY;
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return CHX_ERR_NO_DEVICE;
+  return CHX_ERR_HIP;
+}
+
+// Shape rules (main_wrapper.c:277-278, 564-572 and the kernels' loop
+// steps), enforced here because the reference does not check them.
+static int validate(const chx_config *c) {
+  if (!c) return CHX_ERR_INVALID_ARGUMENT;
+  if (c->kernel_batch < CHX_METHOD_CONFERENCE || c->kernel_batch > CHX_METHOD_INT8_MFMA)
+    return CHX_ERR_INVALID_ARGUMENT;
+  if (c->upper_triangle_convention != 0 && c->upper_triangle_convention != 1)
+    return CHX_ERR_INVALID_ARGUMENT;
+  if (c->num_elements < 32 || c->num_elements % 32 != 0) return CHX_ERR_SHAPE;
+  if (c->num_freq < 1) return CHX_ERR_SHAPE;
+  if (c->time_steps < 32 || c->time_steps % 32 != 0) return CHX_ERR_SHAPE;
+  // Byte count of the input must fit a signed 64-bit size, and the block
+  // buffer a 32-bit grid.
+  const int64_t nb = c->num_elements / 32;
+  if ((int64_t)c->num_freq * nb * (nb + 1) / 2 > 0x7FFFFFFFLL) return CHX_ERR_SHAPE;
+  if (c->time_steps > (INT64_MAX / 2) / ((int64_t)c->num_freq * c->num_elements))
+    return CHX_ERR_SHAPE;
+  const int ta = c->time_accum;
+  if (ta < 8 || ta % 8 != 0 || c->time_steps % ta != 0) return CHX_ERR_TIME_ACCUM;
+  const int32_t cap = chx_max_time_accum(c->kernel_batch);
+  if (cap > 0 && ta > cap) return CHX_ERR_TIME_ACCUM;
+  return CHX_SUCCESS;
+}
+
+}  // namespace chx
+
+using namespace chx;
+
+extern "C" {
+
+const char *chx_version(void) { return "chx 0.1.0 (gfx950)"; }
+
+const char *chx_strerror(int status) {
+  switch (status) {
+    case CHX_SUCCESS: return "success";
+    case CHX_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case CHX_ERR_SHAPE: return "invalid shape (N % 32, N < 32, F < 1, T % 32 or T < 32)";
+    case CHX_ERR_TIME_ACCUM: return "invalid time_accum (% 8, T % time_accum, or over the method cap)";
+    case CHX_ERR_NO_DEVICE: return "no usable GPU device";
+    case CHX_ERR_OUT_OF_MEMORY: return "out of memory";
+    case CHX_ERR_HIP: return "HIP runtime error";
+    case CHX_ERR_BUDGET: return "request exceeds the HBM budget";
+    case CHX_ERR_BUSY: return "no free slot";
+    case CHX_ERR_NOT_FOUND: return "unknown handle";
+    default: return "unknown status";
+  }
+}
+
+const char *chx_last_error_detail(void) { return g_last_error; }
+
+int32_t chx_max_time_accum(int32_t kernel_batch) {
+  switch (kernel_batch) {
+    case CHX_METHOD_CONFERENCE: return CHX_CONFERENCE_MAX_TIME_ACCUM;
+    case CHX_METHOD_HIGHLY_PACKED: return CHX_HIGHLY_PACKED_MAX_TIME_ACCUM;
+    case CHX_METHOD_INT8_MFMA: return 0;
+    default: return -1;
+  }
+}
+
+int chx_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int64_t chx_num_blocks(int32_t num_elements) {
+  if (num_elements < 32 || num_elements % 32) return -1;
+  const int64_t nb = num_elements / 32;
+  return nb * (nb + 1) / 2;
+}
+
+int chx_block_maps(int32_t num_elements, uint32_t *x_map, uint32_t *y_map) {
+  if (!x_map || !y_map || chx_num_blocks(num_elements) < 0) return CHX_ERR_INVALID_ARGUMENT;
+  const int nb = num_elements / 32;
+  int64_t b = 0;
+  for (int by = 0; by < nb; by++)
+    for (int bx = by; bx < nb; bx++, b++) {
+      x_map[b] = (uint32_t)bx;
+      y_map[b] = (uint32_t)by;
+    }
+  return CHX_SUCCESS;
+}
+
+int chx_blocks_to_triangle(int32_t num_freq, int32_t num_elements, const int32_t *blocks,
+                           int32_t *triangle) {
+  if (!blocks || !triangle || num_freq < 1 || chx_num_blocks(num_elements) < 0)
+    return CHX_ERR_INVALID_ARGUMENT;
+  const int N = num_elements, nb = N / 32;
+  const int64_t NB = chx_num_blocks(N), ntri = (int64_t)N * (N + 1) / 2;
+  for (int f = 0; f < num_freq; f++) {
+    int64_t b = 0;
+    for (int by = 0; by < nb; by++)
+      for (int bx = by; bx < nb; bx++, b++) {
+        const int32_t *blk = blocks + ((int64_t)f * NB + b) * 2048;
+        for (int ly = 0; ly < 32; ly++) {
+          const int64_t y = by * 32 + ly;
+          for (int lx = (bx == by ? ly : 0); lx < 32; lx++) {
+            const int64_t x = bx * 32 + lx;
+            const int64_t a = f * ntri + y * N - ((y - 1) * y) / 2 + (x - y);
+            triangle[2 * a] = blk[(ly * 32 + lx) * 2];
+            triangle[2 * a + 1] = blk[(ly * 32 + lx) * 2 + 1];
+          }
+        }
+      }
+  }
+  return CHX_SUCCESS;
+}
+
+int chx_blocks_to_triangle_device(int32_t num_freq, int32_t num_elements,
+                                  const int32_t *d_blocks, int32_t *d_triangle,
+                                  void *stream) {
+  if (!d_blocks || !d_triangle || num_freq < 1 || chx_num_blocks(num_elements) < 0)
+    return CHX_ERR_INVALID_ARGUMENT;
+  return launch_blocks_to_triangle(num_freq, num_elements, d_blocks, d_triangle,
+                                   (hipStream_t)stream);
+}
+
+int chx_plan_create(const chx_config *cfg, chx_plan **out) {
+  if (!out) return CHX_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  int rc = validate(cfg);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return CHX_ERR_NO_DEVICE;
+  if (cfg->device < 0 || cfg->device >= ndev) return CHX_ERR_NO_DEVICE;
+  CHX_HIP(hipSetDevice(cfg->device));
+
+  chx_plan *p = (chx_plan *)calloc(1, sizeof(chx_plan));
+  if (!p) return CHX_ERR_OUT_OF_MEMORY;
+  p->cfg = *cfg;
+  p->nb = cfg->num_elements / 32;
+  p->NB = chx_num_blocks(cfg->num_elements);
+  p->in_bytes = (size_t)cfg->time_steps * cfg->num_freq * cfg->num_elements;
+  p->out_bytes = (size_t)cfg->num_freq * p->NB * 2048 * sizeof(int32_t);
+  p->sums_bytes = (size_t)cfg->num_freq * cfg->num_elements * 2 * sizeof(uint32_t);
+
+  uint32_t *maps = (uint32_t *)malloc(sizeof(uint32_t) * 2 * p->NB);
+  if (!maps) {
+    free(p);
+    return CHX_ERR_OUT_OF_MEMORY;
+  }
+  chx_block_maps(cfg->num_elements, maps, maps + p->NB);
+  hipError_t e = hipMalloc((void **)&p->d_maps, sizeof(uint32_t) * 2 * p->NB);
+  if (e == hipSuccess)
+    e = hipMemcpy(p->d_maps, maps, sizeof(uint32_t) * 2 * p->NB, hipMemcpyHostToDevice);
+  free(maps);
+  if (e == hipSuccess && cfg->kernel_batch != CHX_METHOD_INT8_MFMA)
+    e = hipMalloc((void **)&p->d_sums, p->sums_bytes);
+  if (e != hipSuccess) {
+    rc = hip_status(e);
+    chx_plan_destroy(p);
+    return rc;
+  }
+  *out = p;
+  return CHX_SUCCESS;
+}
+
+int chx_plan_destroy(chx_plan *p) {
+  if (!p) return CHX_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(p->cfg.device);
+  if (p->d_maps) (void)hipFree(p->d_maps);
+  if (p->d_sums) (void)hipFree(p->d_sums);
+  if (p->d_in) (void)hipFree(p->d_in);
+  if (p->d_out) (void)hipFree(p->d_out);
+  free(p);
+  return CHX_SUCCESS;
+}
+
+int chx_plan_get_config(const chx_plan *p, chx_config *cfg) {
+  if (!p || !cfg) return CHX_ERR_INVALID_ARGUMENT;
+  *cfg = p->cfg;
+  return CHX_SUCCESS;
+}
+
+int chx_plan_sizes(const chx_plan *p, size_t *in_b, size_t *out_b, size_t *sums_b) {
+  if (!p) return CHX_ERR_INVALID_ARGUMENT;
+  if (in_b) *in_b = p->in_bytes;
+  if (out_b) *out_b = p->out_bytes;
+  if (sums_b) *sums_b = p->sums_bytes;
+  return CHX_SUCCESS;
+}
+
+int chx_offset_accumulate(const chx_plan *p, const uint8_t *d_input, uint32_t *d_sums,
+                          void *stream) {
+  if (!p || !d_input || !d_sums) return CHX_ERR_INVALID_ARGUMENT;
+  return launch_offset_accumulate(p, d_input, d_sums, (hipStream_t)stream);
+}
+
+int chx_preseed(const chx_plan *p, const uint32_t *d_sums, int32_t *d_vis, void *stream) {
+  if (!p || !d_sums || !d_vis) return CHX_ERR_INVALID_ARGUMENT;
+  if (p->cfg.kernel_batch == CHX_METHOD_INT8_MFMA) return CHX_ERR_INVALID_ARGUMENT;
+  return launch_preseed(p, d_sums, d_vis, (hipStream_t)stream);
+}
+
+int chx_corr(const chx_plan *p, const uint8_t *d_input, int32_t *d_vis, void *stream) {
+  if (!p || !d_input || !d_vis) return CHX_ERR_INVALID_ARGUMENT;
+  if (p->cfg.kernel_batch == CHX_METHOD_INT8_MFMA)
+    return launch_corr_mfma(p, d_input, d_vis, 1, (hipStream_t)stream);
+  return launch_corr_packed(p, d_input, d_vis, (hipStream_t)stream);
+}
+
+int chx_correlate(chx_plan *p, const uint8_t *d_input, int32_t *d_vis, void *stream) {
+  if (!p || !d_input || !d_vis) return CHX_ERR_INVALID_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  if (p->cfg.kernel_batch == CHX_METHOD_INT8_MFMA)
+    return launch_corr_mfma(p, d_input, d_vis, 0, s);
+  int rc = launch_fill_u32(p->d_sums, 0u, (int64_t)p->sums_bytes / 4, s);
+  if (!rc) rc = launch_offset_accumulate(p, d_input, p->d_sums, s);
+  if (!rc) rc = launch_preseed(p, p->d_sums, d_vis, s);
+  if (!rc) rc = launch_corr_packed(p, d_input, d_vis, s);
+  return rc;
+}
+
+int chx_correlate_host(chx_plan *p, const uint8_t *h_input, int32_t *h_vis) {
+  if (!p || !h_input || !h_vis) return CHX_ERR_INVALID_ARGUMENT;
+  CHX_HIP(hipSetDevice(p->cfg.device));
+  if (!p->d_in) CHX_HIP(hipMalloc((void **)&p->d_in, p->in_bytes));
+  if (!p->d_out) CHX_HIP(hipMalloc((void **)&p->d_out, p->out_bytes));
+  CHX_HIP(hipMemcpy(p->d_in, h_input, p->in_bytes, hipMemcpyHostToDevice));
+  int rc = chx_correlate(p, p->d_in, p->d_out, nullptr);
+  if (rc) return rc;
+  CHX_HIP(hipMemcpy(h_vis, p->d_out, p->out_bytes, hipMemcpyDeviceToHost));
+  return CHX_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,160 @@
+/*
+ * chx.h -- C ABI of the MI355X (gfx950) CHIME X-engine correlator.
+ *
+ * Plain C: fixed-width integers, plain pointers and sizes, `void *` for a
+ * hipStream_t (NULL = the device's default stream).  Every entry point
+ * returns an int status (CHX_SUCCESS = 0, like SDK_SUCCESS in the
+ * reference's generator_defs.h:10) and never exits the process.
+ *
+ * Data format (four_bit_macros.h:8-9, input_generator.c:84-96):
+ *   input  [time][freq][element] bytes, real in the high nibble, imaginary in
+ *          the low nibble, offset binary (value = nibble - 8, in [-8, 7]);
+ *   output [freq][block][y 32][x 32][re, im] int32 ("block order"), blocks
+ *          enumerated row-major over the upper triangle of 32x32 element
+ *          blocks (main_wrapper.c:528-540), diagonal blocks complete
+ *          (their lower halves hold the (y > x) products).
+ * Visibility at row y, column x (cpu_corr_test.c:156-203, :59-106):
+ *   re = sum_t x_r y_r + x_i y_i
+ *   im = sum_t x_r y_i - x_i y_r   (upper_triangle_convention = 1)
+ *   im = sum_t x_i y_r - x_r y_i   (upper_triangle_convention = 0)
+ * Sums wrap modulo 2^32 exactly as the reference's int32 accumulation.
+ */
+#ifndef CHX_H
+#define CHX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* kernel_batch values.  0 and 1 are the reference's -k values
+ * (main_wrapper.c:281-305); 2 has no reference counterpart: the int8
+ * matrix-core path, which needs no offset/preseed pass and has no
+ * time_accum cap. */
+#define CHX_METHOD_CONFERENCE 0
+#define CHX_METHOD_HIGHLY_PACKED 1
+#define CHX_METHOD_INT8_MFMA 2
+
+/* time_accum caps of the packed methods: 16-bit lanes gain <= 225 per sample
+ * (README.md:14; 288 is the largest multiple of 8 not above 291), and the
+ * 4-bit overflow counters of the highly-packed method
+ * (packed_correlator_overflow_protected_to_2272_iter_UT.cl:171). */
+#define CHX_CONFERENCE_MAX_TIME_ACCUM 288
+#define CHX_HIGHLY_PACKED_MAX_TIME_ACCUM 2272
+
+enum chx_status {
+  CHX_SUCCESS = 0,
+  CHX_ERR_INVALID_ARGUMENT = 1, /* NULL pointer, bad enum value */
+  CHX_ERR_SHAPE = 2,            /* N % 32, N < 32, F < 1, T % 32, T < 32 */
+  CHX_ERR_TIME_ACCUM = 3,       /* ta % 8, T % ta, ta over the method cap */
+  CHX_ERR_NO_DEVICE = 4,        /* no GPU, or device index out of range */
+  CHX_ERR_OUT_OF_MEMORY = 5,    /* device or pinned allocation failed */
+  CHX_ERR_HIP = 6,              /* any other HIP runtime error */
+  CHX_ERR_BUDGET = 7,           /* request exceeds a configured HBM budget */
+  CHX_ERR_BUSY = 8,             /* no free slot (non-blocking admission) */
+  CHX_ERR_NOT_FOUND = 9         /* unknown job/stream handle */
+};
+
+/* Replaces the -D shape macros of the runtime-compiled OpenCL program
+ * (main_wrapper.c:309-312) and the CLI values that pick it
+ * (main_wrapper.c:84-103, 281-305). */
+typedef struct chx_config {
+  int32_t num_elements;              /* N: NUM_ELEMENTS */
+  int32_t num_freq;                  /* F: NUM_FREQUENCIES */
+  int64_t time_steps;                /* T: NUM_TIMESAMPLES (64-bit here) */
+  int32_t time_accum;                /* NUM_TIME_ACCUM */
+  int32_t upper_triangle_convention; /* -U: 1 standard, 0 conjugate */
+  int32_t kernel_batch;              /* -k: CHX_METHOD_* */
+  int32_t device;                    /* -d */
+} chx_config;
+
+typedef struct chx_plan chx_plan;
+
+/* Validates cfg and builds a plan: replaces context/queue/program/kernel
+ * creation and the block id maps (main_wrapper.c:238-372, 528-552).  On a
+ * validation error nothing touches the GPU. */
+int chx_plan_create(const chx_config *cfg, chx_plan **plan);
+int chx_plan_destroy(chx_plan *plan);
+int chx_plan_get_config(const chx_plan *plan, chx_config *cfg);
+
+/* Bytes of the input ([T][F][N] u8), of the block-order output
+ * (F * NB * 2048 int32), and of the offset sums (F * N * 2 uint32). */
+int chx_plan_sizes(const chx_plan *plan, size_t *input_bytes,
+                   size_t *output_bytes, size_t *sums_bytes);
+
+/* ---- Stage operations on device pointers (one per reference kernel). ---- */
+
+/* offsetAccumulateElements (offset_accumulator.cl:7-49): adds
+ * (sum_t R, sum_t I) of the raw nibbles into d_sums[F][N][2]; the caller
+ * zeroes d_sums first, as main_wrapper.c:671-679 does. */
+int chx_offset_accumulate(const chx_plan *plan, const uint8_t *d_input,
+                          uint32_t *d_sums, void *stream);
+
+/* preseed (preseed_multifreq{,_UT}.cl:19-116 and
+ * preseed_multifreq_highly_packed_correlator_method{,_UT}.cl): assigns the
+ * offset-binary correction of the plan's method and convention to every
+ * entry of d_vis.  Methods 0 and 1 only. */
+int chx_preseed(const chx_plan *plan, const uint32_t *d_sums, int32_t *d_vis,
+                void *stream);
+
+/* corr (pairwise_correlator{,_UT}.cl:22-207,
+ * packed_correlator_overflow_protected_to_2272_iter{,_UT}.cl): adds the
+ * method's products into d_vis.  Methods 0/1 add the raw nibble-space
+ * products (d_vis must hold the preseed); method 2 adds the finished signed
+ * visibilities.  No lock buffer: blocks are owned, split time ranges add
+ * with atomics. */
+int chx_corr(const chx_plan *plan, const uint8_t *d_input, int32_t *d_vis,
+             void *stream);
+
+/* The whole per-iteration pipeline (main_wrapper.c:691-778): assigns the
+ * final visibilities of d_input to d_vis.  Methods 0/1: zero sums ->
+ * offset accumulate -> preseed -> corr, with plan-owned sums; method 2: one
+ * matrix-core pass. */
+int chx_correlate(chx_plan *plan, const uint8_t *d_input, int32_t *d_vis,
+                  void *stream);
+
+/* Host-pointer convenience: copies h_input in, correlates, copies the block
+ * buffer out; synchronous.  Device buffers are owned by the plan. */
+int chx_correlate_host(chx_plan *plan, const uint8_t *h_input, int32_t *h_vis);
+
+/* ---- Layout helpers (host only, no GPU needed). ---- */
+
+/* NB = (N/32)(N/32+1)/2 (main_wrapper.c:277). Returns -1 on a bad N. */
+int64_t chx_num_blocks(int32_t num_elements);
+
+/* id_x_map / id_y_map (main_wrapper.c:528-540). */
+int chx_block_maps(int32_t num_elements, uint32_t *x_map, uint32_t *y_map);
+
+/* reorganize_GPU_to_upper_triangle (gpu_data_reorg.c:76-124): block order
+ * -> [F][N(N+1)/2][2] packed upper triangle, row-major. */
+int chx_blocks_to_triangle(int32_t num_freq, int32_t num_elements,
+                           const int32_t *blocks, int32_t *triangle);
+
+/* Device version of the same reorg (no reference counterpart: it keeps the
+ * visibilities on the GPU for consumers that want the triangle). */
+int chx_blocks_to_triangle_device(int32_t num_freq, int32_t num_elements,
+                                  const int32_t *d_blocks, int32_t *d_triangle,
+                                  void *stream);
+
+/* Replaces oclGetOpenCLErrorCodeStr
+ * (amd_firepro_error_code_list_for_opencl.c:4-132). */
+const char *chx_strerror(int status);
+/* Text of the last HIP error seen by this thread, or "" if none. */
+const char *chx_last_error_detail(void);
+
+/* Largest time_accum a method accepts (0 = no cap), -1 on a bad method. */
+int32_t chx_max_time_accum(int32_t kernel_batch);
+
+/* Number of visible GPUs (0 on a machine without one). */
+int chx_device_count(void);
+
+/* Library version string. */
+const char *chx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHX_H */

@@ -1,0 +1,101 @@
+"""The C ABI without a GPU: the library loads, exports every symbol the
+header declares, its host helpers match the oracle / reference fixtures, and
+bad shapes are refused before anything touches a device."""
+import ctypes
+import subprocess
+
+import numpy as np
+import pytest
+
+import chx
+import oracle as O
+
+
+def test_exports_every_header_symbol():
+    syms = chx.header_symbols()
+    assert len(syms) >= 18
+    out = subprocess.run(["nm", "-D", "--defined-only", chx.LIB_PATH], check=True,
+                         capture_output=True, text=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = [s for s in syms if s not in exported]
+    assert not missing, missing
+    for s in syms:
+        assert hasattr(chx.lib(), s)
+
+
+def test_num_blocks_and_maps():
+    for N in (32, 64, 256, 2048, 4096):
+        assert chx.num_blocks(N) == O.num_blocks(N)
+        xm, ym = chx.block_maps(N)
+        ox, oy = O.block_maps(N)
+        np.testing.assert_array_equal(xm, ox)
+        np.testing.assert_array_equal(ym, oy)
+    assert chx.num_blocks(48) == -1 and chx.num_blocks(0) == -1
+
+
+def test_blocks_to_triangle_matches_reference_reorg(golden):
+    for N, F in ((96, 2), (32, 1), (256, 1)):
+        nb = chx.num_blocks(N)
+        blocks = np.arange(F * nb * 2048, dtype=np.int32)
+        np.testing.assert_array_equal(chx.blocks_to_triangle(blocks, N),
+                                      golden[f"reorg_N{N}_F{F}__tri"])
+
+
+def test_strerror_and_caps():
+    for s in range(0, 10):
+        assert chx.lib().chx_strerror(s)
+    assert chx.lib().chx_max_time_accum(0) == 288
+    assert chx.lib().chx_max_time_accum(1) == 2272
+    assert chx.lib().chx_max_time_accum(2) == 0
+    assert chx.lib().chx_max_time_accum(7) == -1
+    assert b"gfx950" in chx.lib().chx_version()
+
+
+def _create(**kw):
+    base = dict(num_elements=256, num_freq=1, time_steps=256, time_accum=256,
+                upper_triangle_convention=1, kernel_batch=0, device=0)
+    base.update(kw)
+    cfg = chx.Config(**base)
+    h = ctypes.c_void_p()
+    rc = chx.lib().chx_plan_create(ctypes.byref(cfg), ctypes.byref(h))
+    if rc == 0:
+        chx.lib().chx_plan_destroy(h)
+    return rc
+
+
+@pytest.mark.parametrize("kw,status", [
+    (dict(num_elements=48), chx.ERR_SHAPE),
+    (dict(num_elements=0), chx.ERR_SHAPE),
+    (dict(num_freq=0), chx.ERR_SHAPE),
+    (dict(time_steps=272, time_accum=8), chx.ERR_SHAPE),     # T % 32
+    (dict(time_steps=0), chx.ERR_SHAPE),
+    (dict(time_accum=12), chx.ERR_TIME_ACCUM),               # ta % 8
+    (dict(time_steps=320, time_accum=256), chx.ERR_TIME_ACCUM),  # T % ta
+    (dict(time_steps=296 * 4, time_accum=296), chx.ERR_TIME_ACCUM),  # conference cap
+    (dict(kernel_batch=1, time_steps=2280 * 4, time_accum=2280), chx.ERR_TIME_ACCUM),
+    (dict(kernel_batch=3), chx.ERR_INVALID_ARGUMENT),
+    (dict(upper_triangle_convention=2), chx.ERR_INVALID_ARGUMENT),
+    (dict(num_freq=2 ** 20, num_elements=4096), chx.ERR_SHAPE),  # grid bound
+])
+def test_validation_is_host_side(kw, status):
+    assert _create(**kw) == status
+
+
+def test_null_arguments():
+    L = chx.lib()
+    assert L.chx_plan_create(None, None) == chx.ERR_INVALID_ARGUMENT
+    h = ctypes.c_void_p()
+    assert L.chx_plan_create(None, ctypes.byref(h)) == chx.ERR_INVALID_ARGUMENT
+    assert L.chx_plan_destroy(None) == chx.ERR_INVALID_ARGUMENT
+    assert L.chx_correlate(None, None, None, None) == chx.ERR_INVALID_ARGUMENT
+    assert L.chx_blocks_to_triangle(1, 64, None, None) == chx.ERR_INVALID_ARGUMENT
+
+
+def test_valid_config_without_gpu_reports_no_device():
+    if chx.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    assert _create() == chx.ERR_NO_DEVICE
+    assert _create(kernel_batch=2, time_accum=8) == chx.ERR_NO_DEVICE
+    with pytest.raises(chx.ChxError) as e:
+        chx.Plan(256, 1, 256)
+    assert e.value.status == chx.ERR_NO_DEVICE

@@ -1,0 +1,181 @@
+"""Bit-exact parity of the HIP paths against the oracle, through the C ABI.
+
+Every method (conference, highly packed, int8 MFMA) x both conventions is
+checked on: the reference's own CPU-correlator fixtures (config 1: N=256,
+F=1, T=256), seeded random inputs of ragged shapes against the oracle's full
+block buffer (diagonal lower halves included), the stage operations one by
+one, the worst-case lane data at each method's time_accum cap, and
+closed forms (saturation, conjugation, linearity in T).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import chx
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+METHODS = [chx.CONFERENCE, chx.HIGHLY_PACKED, chx.INT8_MFMA]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run(data, method, U, ta):
+    T, F, N = data.shape
+    say(f"  case method={chx.METHOD_NAMES[method]} U={U} T={T} F={F} N={N} ta={ta}")
+    with chx.Plan(N, F, T, ta, U, method) as p:
+        return p.correlate(data)
+
+
+def rand(T, F, N, seed):
+    return np.random.default_rng(seed).integers(0, 256, size=(T, F, N), dtype=np.uint8)
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("U", [1, 0])
+@pytest.mark.parametrize("case", ["c1_repeat", "c1_continuous"])
+def test_config1_matches_reference_cpu_correlator(golden, method, U, case):
+    data = golden[f"{case}__input"]
+    blocks = run(data, method, U, 256)
+    tri = chx.blocks_to_triangle(blocks, 256)
+    np.testing.assert_array_equal(tri, golden[f"{case}__tri_U{U}"])
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("U", [1, 0])
+def test_reference_fixture_shapes(golden, method, U):
+    for case in ("rand_cont_s7", "single_freq_1"):
+        data = golden[f"{case}__input"]
+        T = data.shape[0]
+        blocks = run(data, method, U, 32 if T % 64 else 64)
+        np.testing.assert_array_equal(chx.blocks_to_triangle(blocks, data.shape[2]),
+                                      golden[f"{case}__tri_U{U}"], err_msg=case)
+
+
+SHAPES = [  # (T, F, N, ta)
+    (32, 1, 32, 8),
+    (64, 2, 64, 32),
+    (256, 3, 96, 128),
+    (288, 1, 160, 288),
+    (1024, 2, 128, 256),
+    (2304, 1, 256, 1152),
+    (512, 1, 384, 64),
+]
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("U", [1, 0])
+def test_random_shapes_against_oracle(method, U):
+    for i, (T, F, N, ta) in enumerate(SHAPES):
+        if method == chx.CONFERENCE and ta > 288:
+            ta = 288 if T % 288 == 0 else 256
+            if T % ta:
+                ta = 64
+        data = rand(T, F, N, 1000 + i)
+        np.testing.assert_array_equal(run(data, method, U, ta), O.correlate_blocks(data, U),
+                                      err_msg=f"shape {T, F, N, ta}")
+
+
+@pytest.mark.parametrize("method,ta", [(chx.CONFERENCE, 288), (chx.HIGHLY_PACKED, 2272),
+                                       (chx.INT8_MFMA, 2272)])
+def test_worst_case_lanes_at_cap(method, ta):
+    T = ta * 2
+    data = np.full((T, 1, 64), 0xFF, np.uint8)
+    data[:, :, 1::2] = 0xF0
+    data[T // 2:, :, ::3] = 0x0F
+    for U in (1, 0):
+        np.testing.assert_array_equal(run(data, method, U, ta), O.correlate_blocks(data, U))
+
+
+def _torch():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("torch sees no GPU")
+    return torch
+
+
+@pytest.mark.parametrize("method", [chx.CONFERENCE, chx.HIGHLY_PACKED])
+@pytest.mark.parametrize("U", [1, 0])
+def test_stage_operations(method, U):
+    torch = _torch()
+    T, F, N, ta = 512, 2, 96, 128
+    data = rand(T, F, N, 77)
+    say(f"  stages method={method} U={U}")
+    with chx.Plan(N, F, T, ta, U, method) as p:
+        d_in = torch.from_numpy(data).cuda()
+        d_sums = torch.zeros((F, N, 2), dtype=torch.int32, device="cuda")
+        d_vis = torch.zeros(p.output_shape, dtype=torch.int32, device="cuda")
+        s = torch.cuda.current_stream().cuda_stream
+        p.offset_accumulate(d_in.data_ptr(), d_sums.data_ptr(), s)
+        torch.cuda.synchronize()
+        sums = d_sums.cpu().numpy().view(np.uint32)
+        np.testing.assert_array_equal(sums, O.offset_sums(data))
+        p.preseed(d_sums.data_ptr(), d_vis.data_ptr(), s)
+        torch.cuda.synchronize()
+        pre = O.preseed(sums, T, method, U)
+        np.testing.assert_array_equal(d_vis.cpu().numpy(), pre)
+        p.corr(d_in.data_ptr(), d_vis.data_ptr(), s)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(d_vis.cpu().numpy(), O.correlate_blocks(data, U))
+        # raw products alone == the packed-lane emulation
+        d_vis.zero_()
+        p.corr(d_in.data_ptr(), d_vis.data_ptr(), s)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(d_vis.cpu().numpy(), O.packed_raw(data, method, U, ta))
+
+
+def test_mfma_corr_adds_and_device_triangle():
+    torch = _torch()
+    T, F, N = 256, 2, 128
+    data = rand(T, F, N, 5)
+    with chx.Plan(N, F, T, 256, 1, chx.INT8_MFMA) as p:
+        d_in = torch.from_numpy(data).cuda()
+        d_vis = torch.full(p.output_shape, 7, dtype=torch.int32, device="cuda")
+        p.corr(d_in.data_ptr(), d_vis.data_ptr())
+        torch.cuda.synchronize()
+        want = O.correlate_blocks(data, 1)
+        np.testing.assert_array_equal(d_vis.cpu().numpy(), want + 7)
+        p.correlate_device(d_in.data_ptr(), d_vis.data_ptr())
+        d_tri = torch.empty((F, N * (N + 1) // 2, 2), dtype=torch.int32, device="cuda")
+        chx.blocks_to_triangle_device(F, N, d_vis.data_ptr(), d_tri.data_ptr())
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(d_tri.cpu().numpy(), O.blocks_to_triangle(want, N))
+
+
+def test_mfma_long_integration_saturation_closed_form():
+    # Every sample (-8,-8): re = 128*T, im = 0, over several 32768-sample
+    # accumulator chunks.
+    T, F, N = 131072, 2, 128
+    data = np.zeros((T, F, N), np.uint8)
+    blocks = run(data, chx.INT8_MFMA, 1, 256)
+    assert (blocks[..., 0] == 128 * T).all() and (blocks[..., 1] == 0).all()
+
+
+def test_mfma_time_split_and_linearity():
+    # Small N with long T takes the split-time (atomic) path.
+    T, F, N = 16384, 1, 64
+    data = rand(T, F, N, 9)
+    full = run(data, chx.INT8_MFMA, 1, 256)
+    a = run(np.ascontiguousarray(data[:8192]), chx.INT8_MFMA, 1, 256)
+    b = run(np.ascontiguousarray(data[8192:]), chx.INT8_MFMA, 1, 256)
+    np.testing.assert_array_equal(full, a + b)
+    np.testing.assert_array_equal(full, O.correlate_blocks(data, 1))
+    conj = run(data, chx.INT8_MFMA, 0, 256)
+    np.testing.assert_array_equal(conj[..., 0], full[..., 0])
+    np.testing.assert_array_equal(conj[..., 1], -full[..., 1])
+
+
+def test_methods_agree_on_array_scale_channel_slice():
+    # N=2048 (config 2's array) on a T slice: all three methods bit-exact
+    # against each other and against the oracle.
+    T, F, N = 512, 2, 2048
+    data = rand(T, F, N, 2048)
+    want = O.correlate_blocks(data, 1)
+    for method, ta in ((chx.INT8_MFMA, 512), (chx.CONFERENCE, 256), (chx.HIGHLY_PACKED, 512)):
+        np.testing.assert_array_equal(run(data, method, 1, ta), want,
+                                      err_msg=chx.METHOD_NAMES[method])
