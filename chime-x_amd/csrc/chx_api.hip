@@ -232,13 +232,51 @@ int chx_correlate(chx_plan *p, const uint8_t *d_input, int32_t *d_vis, void *str
   return rc;
 }
 
-int chx_correlate_host(chx_plan *p, const uint8_t *h_input, int32_t *h_vis) {
-  if (!p || !h_input || !h_vis) return CHX_ERR_INVALID_ARGUMENT;
+int chx_plan_buffers(chx_plan *p, uint8_t **d_input, int32_t **d_vis) {
+  if (!p) return CHX_ERR_INVALID_ARGUMENT;
   CHX_HIP(hipSetDevice(p->cfg.device));
   if (!p->d_in) CHX_HIP(hipMalloc((void **)&p->d_in, p->in_bytes));
   if (!p->d_out) CHX_HIP(hipMalloc((void **)&p->d_out, p->out_bytes));
+  if (d_input) *d_input = p->d_in;
+  if (d_vis) *d_vis = p->d_out;
+  return CHX_SUCCESS;
+}
+
+int chx_host_alloc(void **ptr, size_t bytes) {
+  if (!ptr) return CHX_ERR_INVALID_ARGUMENT;
+  CHX_HIP(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+  return CHX_SUCCESS;
+}
+
+int chx_host_free(void *ptr) {
+  if (!ptr) return CHX_ERR_INVALID_ARGUMENT;
+  CHX_HIP(hipHostFree(ptr));
+  return CHX_SUCCESS;
+}
+
+int chx_copy_to_device(void *d_dst, const void *h_src, size_t bytes, void *stream) {
+  if (!d_dst || !h_src) return CHX_ERR_INVALID_ARGUMENT;
+  CHX_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+  return CHX_SUCCESS;
+}
+
+int chx_copy_to_host(void *h_dst, const void *d_src, size_t bytes, void *stream) {
+  if (!h_dst || !d_src) return CHX_ERR_INVALID_ARGUMENT;
+  CHX_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  return CHX_SUCCESS;
+}
+
+int chx_stream_synchronize(void *stream) {
+  CHX_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return CHX_SUCCESS;
+}
+
+int chx_correlate_host(chx_plan *p, const uint8_t *h_input, int32_t *h_vis) {
+  if (!p || !h_input || !h_vis) return CHX_ERR_INVALID_ARGUMENT;
+  int rc = chx_plan_buffers(p, nullptr, nullptr);
+  if (rc) return rc;
   CHX_HIP(hipMemcpy(p->d_in, h_input, p->in_bytes, hipMemcpyHostToDevice));
-  int rc = chx_correlate(p, p->d_in, p->d_out, nullptr);
+  rc = chx_correlate(p, p->d_in, p->d_out, nullptr);
   if (rc) return rc;
   CHX_HIP(hipMemcpy(h_vis, p->d_out, p->out_bytes, hipMemcpyDeviceToHost));
   return CHX_SUCCESS;

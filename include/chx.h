@@ -119,6 +119,21 @@ int chx_correlate(chx_plan *plan, const uint8_t *d_input, int32_t *d_vis,
  * buffer out; synchronous.  Device buffers are owned by the plan. */
 int chx_correlate_host(chx_plan *plan, const uint8_t *h_input, int32_t *h_vis);
 
+/* ---- Plan-owned device buffers and copies (for C hosts that do not link
+ * HIP themselves; replace clCreateBuffer / clEnqueueWrite/ReadBuffer,
+ * main_wrapper.c:379-501, 655-667, 798-799). ---- */
+
+/* Allocates (once) and returns the plan's device input and output buffers. */
+int chx_plan_buffers(chx_plan *plan, uint8_t **d_input, int32_t **d_vis);
+/* Pinned host memory (hipHostMalloc), for fast H2D of large inputs. */
+int chx_host_alloc(void **ptr, size_t bytes);
+int chx_host_free(void *ptr);
+/* Asynchronous copies on stream (NULL = default); dst/src by direction. */
+int chx_copy_to_device(void *d_dst, const void *h_src, size_t bytes, void *stream);
+int chx_copy_to_host(void *h_dst, const void *d_src, size_t bytes, void *stream);
+/* Waits for all work queued on stream (NULL = default stream). */
+int chx_stream_synchronize(void *stream);
+
 /* ---- Layout helpers (host only, no GPU needed). ---- */
 
 /* NB = (N/32)(N/32+1)/2 (main_wrapper.c:277). Returns -1 on a bad N. */
