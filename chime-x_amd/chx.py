@@ -8,7 +8,13 @@ pairwise_correlator_UT.cl:22) and the host loop that chains them
 input [time][freq][element], block-order int32 output, the
 upper_triangle_convention flag and time_accum.
 
-There is no CPU fallback: if libchx.so is missing, importing raises.
+There is no CPU fallback: if libchx.so is missing, lib() raises.
+
+Processes that also use torch must import torch BEFORE the first lib()
+call: torch ships its own HIP runtime (libamdhip64.so without a SONAME);
+loaded first, it becomes the runtime libchx binds to, so torch's device
+pointers and streams are valid arguments of the C ABI.  Loaded after,
+the process would hold two HIP runtimes.
 """
 from __future__ import annotations
 
@@ -20,7 +26,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
-LIB_PATH = os.path.join(HERE, "libchx.so")
+LIB_PATH = os.environ.get("CHX_LIB", os.path.join(HERE, "libchx.so"))
 HEADER = os.path.join(REPO, "include", "chx.h")
 
 CONFERENCE, HIGHLY_PACKED, INT8_MFMA = 0, 1, 2

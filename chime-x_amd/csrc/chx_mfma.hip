@@ -30,12 +30,17 @@
 namespace chx {
 namespace {
 
+typedef int v2i __attribute__((ext_vector_type(2)));
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int kTile = 128;       // elements per tile side
 constexpr int kK = 32;           // samples per stage (MFMA K)
 constexpr int kChunk = 32768;    // samples accumulated before a flush
+#ifndef CHX_MFMA_RING
+#define CHX_MFMA_RING 4
+#endif
+constexpr int kRing = CHX_MFMA_RING;  // stages of raw input in flight per thread
 constexpr int kRows = 2 * kTile; // LDS rows per plane: row tile, then column tile
 // LDS row of element row e: one spare 32-byte row after every 8 keeps both the
 // 8-byte transposed writes and the 16-byte operand reads free of bank conflicts.
@@ -80,23 +85,27 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t id, int64_t total) {
 // layout).  mode 0 assigns, 1 adds, 2 adds atomically.
 __device__ __forceinline__ void flush_block(int32_t *blk, const v16i &re, const v16i &ix,
                                             const v16i &iy, int U, int mode, int lane) {
-  const int n = lane & 31, mhi = (lane >> 5) * 4;
+  // Buffer ops on a descriptor of the block (a wave-uniform base): one VGPR
+  // of lane offset, the register's row as a constant scalar offset.
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc(blk, (short)0, 2048 * 4, 0x00020000);
+  const int voff = ((lane >> 5) * 4 * 32 + (lane & 31)) * 8;
 #pragma unroll
   for (int j = 0; j < 16; j++) {
-    const int m = 8 * (j >> 2) + mhi + (j & 3);
     const int32_t vr = re[j] >> 8;
     const int32_t vi = U ? (ix[j] - iy[j]) >> 8 : (iy[j] - ix[j]) >> 8;
-    int32_t *o = blk + (m * 32 + n) * 2;
+    const int soff = (8 * (j >> 2) + (j & 3)) * 32 * 8;  // row m = 8(j/4) + 4(l/32) + j%4
     if (mode == 0) {
-      *(int2 *)o = make_int2(vr, vi);
+      v2i v = {vr, vi};
+      __builtin_amdgcn_raw_buffer_store_b64(v, rsrc, voff, soff, 0);
     } else if (mode == 1) {
-      int2 a = *(int2 *)o;
+      v2i a = __builtin_amdgcn_raw_buffer_load_b64(rsrc, voff, soff, 0);
       a.x += vr;
       a.y += vi;
-      *(int2 *)o = a;
+      __builtin_amdgcn_raw_buffer_store_b64(a, rsrc, voff, soff, 0);
     } else {
-      atomicAdd(o, vr);
-      atomicAdd(o + 1, vi);
+      __builtin_amdgcn_raw_ptr_buffer_atomic_add_i32(vr, rsrc, voff, soff, 0);
+      __builtin_amdgcn_raw_ptr_buffer_atomic_add_i32(vi, rsrc, voff + 4, soff, 0);
     }
   }
 }
@@ -104,17 +113,25 @@ __device__ __forceinline__ void flush_block(int32_t *blk, const v16i &re, const 
 // grid.x = channels x upper-triangle tiles (XCD-remapped), grid.y = time
 // splits.  mode: 0 assign, 1 add (applies to the first flush; later flushes
 // of the same workgroup add); splits > 1 always add atomically.
+//
+// Pipeline: the raw words of stage s sit in register ring slot s % D, loaded
+// D stages before they are transposed into LDS, so each load has D stages of
+// matrix work (D * 512 cycles) to arrive; LDS planes are double-buffered, one
+// barrier per stage.
+template <int D>
 __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(
     const uint8_t *__restrict__ in, int32_t *__restrict__ vis, int N, int F,
     int64_t row_bytes, int64_t t_split, int ntiles, int64_t NB, int nb, int U,
     int mode) {
+  static_assert((kChunk / kK) % D == 0, "ring depth");
   __shared__ __attribute__((aligned(16))) uint8_t lds[2][2][kPlaneBytes];  // [buf][re/im]
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t total = (int64_t)gridDim.x;
   const int64_t item = xcd_remap(blockIdx.x, total);
-  const int f = (int)(item / ((int64_t)ntiles * (ntiles + 1) / 2));
-  int tb = (int)(item % ((int64_t)ntiles * (ntiles + 1) / 2));
+  const int64_t per_f = (int64_t)ntiles * (ntiles + 1) / 2;
+  const int f = (int)(item / per_f);
+  int tb = (int)(item % per_f);
   int ti = 0;
   while (tb >= ntiles - ti) {
     tb -= ntiles - ti;
@@ -124,12 +141,13 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(
   const int64_t t_begin = (int64_t)blockIdx.y * t_split;
 
   // Loader role: wave w loads quads 16w..16w+15 (quads 0-31: row tile,
-  // 32-63: column tile), lane = (quad, 8-sample group).
+  // 32-63: column tile), lane = (quad, 8-sample group).  Quads beyond N read
+  // element 0 of the same row (always in bounds) and are replaced by the
+  // byte 0x88, the value (0, 0).
   const int lq = 16 * wave + (lane >> 2), lg = lane & 3;
   const int le = (lq & 31) * 4;                       // element within its tile
   const int lelem = (lq < 32 ? ti : tj) * kTile + le; // absolute element
   const bool lvalid = lelem < N;
-  const uint8_t *src = in + (t_begin + 8 * lg) * row_bytes + (int64_t)f * N + lelem;
   const int lrow = (lq < 32 ? 0 : kTile) + le;        // first LDS row of the quad
 
   // Compute role: wave (wr, wc) = rows 64*wr.., columns 64*wc.. of the tile.
@@ -149,15 +167,24 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(
 #pragma unroll
     for (int j = 0; j < 2; j++) acc_re[i][j] = acc_ix[i][j] = acc_iy[i][j] = zero;
 
-  uint32_t w[8];
-  auto load_stage = [&](int64_t ts) {
-    const uint8_t *p = src + ts * row_bytes;
+  uint32_t ring[D][8];
+  // Buffer loads: the descriptor is rebased on the stage's first row (a
+  // scalar), the lane offset is one VGPR and the sample row is the scalar
+  // offset, so the eight loads of a stage need no 64-bit address math.
+  const uint8_t *chan = in + t_begin * row_bytes + (int64_t)f * N;
+  const int voff = 8 * lg * (int)row_bytes + (lvalid ? lelem : 0);
+  const int rows_bytes = (int)(kK * row_bytes);
+  auto load_stage = [&](uint32_t (&w)[8], int64_t st) {
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(chan + st * kK * row_bytes), (short)0, rows_bytes, 0x00020000);
 #pragma unroll
     for (int s = 0; s < 8; s++)
-      w[s] = lvalid ? *(const uint32_t *)(p + s * row_bytes) : 0x88888888u;
+      w[s] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, s * (int)row_bytes, 0);
   };
-  auto store_stage = [&](int buf) {
+  auto store_stage = [&](uint32_t (&w)[8], int buf) {
     uint32_t lo[4], hi[4];
+#pragma unroll
+    for (int s = 0; s < 8; s++) w[s] = lvalid ? w[s] : 0x88888888u;
     transpose4(w[0], w[1], w[2], w[3], lo[0], lo[1], lo[2], lo[3]);
     transpose4(w[4], w[5], w[6], w[7], hi[0], hi[1], hi[2], hi[3]);
 #pragma unroll
@@ -170,40 +197,54 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(
       *(uint2 *)&lds[buf][1][off] = make_uint2(il, ih);
     }
   };
+  auto compute_stage = [&](int buf) {
+    v4i ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      ar[i] = *(const v4i *)&lds[buf][0][aoff[i]];
+      ai[i] = *(const v4i *)&lds[buf][1][aoff[i]];
+      br[i] = *(const v4i *)&lds[buf][0][boff[i]];
+      bi[i] = *(const v4i *)&lds[buf][1][boff[i]];
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        acc_re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[i], br[j], acc_re[i][j], 0, 0, 0);
+        acc_re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ai[i], bi[j], acc_re[i][j], 0, 0, 0);
+        acc_ix[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ai[i], br[j], acc_ix[i][j], 0, 0, 0);
+        acc_iy[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[i], bi[j], acc_iy[i][j], 0, 0, 0);
+      }
+  };
 
   const int64_t nst = t_split / kK;
   int cur_mode = mode;
-
   // Output blocks of this wave (upper triangle, inside N).
   const int by0 = (ti * kTile + 64 * wr) / 32, bx0 = (tj * kTile + 64 * wc) / 32;
 
-  load_stage(0);
-  store_stage(0);
+  // Every load is issued unconditionally (stage index clamped into range)
+  // so the wait counts stay exact: stage st+1's words are the oldest D*8
+  // loads in flight when they are consumed.
+  const int64_t last = nst - 1;
+#pragma unroll
+  for (int d = 0; d < D; d++) load_stage(ring[d], d < last ? d : last);
+  store_stage(ring[0], 0);
+  load_stage(ring[0], D < last ? D : last);
   __syncthreads();
+
   for (int64_t c0 = 0; c0 < nst; c0 += kChunk / kK) {
     const int64_t c1 = c0 + kChunk / kK < nst ? c0 + kChunk / kK : nst;
-    for (int64_t st = c0; st < c1; st++) {
-      const int buf = (int)(st & 1);
-      if (st + 1 < nst) load_stage((st + 1) * kK);
-      v4i ar[2], ai[2], br[2], bi[2];
+    for (int64_t st0 = c0; st0 < c1; st0 += D) {
 #pragma unroll
-      for (int i = 0; i < 2; i++) {
-        ar[i] = *(const v4i *)&lds[buf][0][aoff[i]];
-        ai[i] = *(const v4i *)&lds[buf][1][aoff[i]];
-        br[i] = *(const v4i *)&lds[buf][0][boff[i]];
-        bi[i] = *(const v4i *)&lds[buf][1][boff[i]];
+      for (int d = 0; d < D; d++) {
+        const int64_t st = st0 + d;
+        const int buf = (int)(st & 1);
+        compute_stage(buf);
+        if (st + 1 < nst) store_stage(ring[(d + 1) % D], buf ^ 1);
+        const int64_t nxt = st + 1 + D;
+        load_stage(ring[(d + 1) % D], nxt < last ? nxt : last);
+        __syncthreads();
       }
-#pragma unroll
-      for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-          acc_re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[i], br[j], acc_re[i][j], 0, 0, 0);
-          acc_re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ai[i], bi[j], acc_re[i][j], 0, 0, 0);
-          acc_ix[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ai[i], br[j], acc_ix[i][j], 0, 0, 0);
-          acc_iy[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[i], bi[j], acc_iy[i][j], 0, 0, 0);
-        }
-      if (st + 1 < nst) store_stage(buf ^ 1);
-      __syncthreads();
     }
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -246,9 +287,18 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
     kmode = 2;
   }
   if (items > 0x7FFFFFFF) return CHX_ERR_SHAPE;
-  corr_mfma_kernel<<<dim3((unsigned)items, (unsigned)split), dim3(256), 0, s>>>(
-      in, vis, N, F, (int64_t)F * N, t_split, ntiles, p->NB, p->nb,
-      p->cfg.upper_triangle_convention, kmode);
+  const dim3 grid((unsigned)items, (unsigned)split), blk(256);
+  const int64_t nst = t_split / kK;
+  const int U = p->cfg.upper_triangle_convention;
+  const int64_t rb = (int64_t)F * N;
+  // The stage loop runs in groups of the ring depth: pick the deepest ring
+  // that divides the per-split stage count.
+  if (nst % kRing == 0)
+    corr_mfma_kernel<kRing><<<grid, blk, 0, s>>>(in, vis, N, F, rb, t_split, ntiles, p->NB, p->nb, U, kmode);
+  else if (nst % 2 == 0)
+    corr_mfma_kernel<2><<<grid, blk, 0, s>>>(in, vis, N, F, rb, t_split, ntiles, p->NB, p->nb, U, kmode);
+  else
+    corr_mfma_kernel<1><<<grid, blk, 0, s>>>(in, vis, N, F, rb, t_split, ntiles, p->NB, p->nb, U, kmode);
   CHX_HIP(hipGetLastError());
   return CHX_SUCCESS;
 }

@@ -4,6 +4,14 @@ import sys
 import numpy as np
 import pytest
 
+# torch ships its own HIP runtime (libamdhip64.so without a SONAME).  Loading
+# it before libchx.so makes libchx bind to that same runtime instance, so
+# device pointers and streams from torch can be handed to the C ABI.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    torch = None
+
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(REPO, "oracle"))
 sys.path.insert(0, os.path.join(REPO, "chime-x_amd"))
