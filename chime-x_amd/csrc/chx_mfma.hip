@@ -23,8 +23,9 @@
 // accumulators of 16 registers each per block), v_mfma_i32_32x32x32_i8 with
 // K = 32 samples per stage.  Stages are loaded as [time][element] bytes,
 // transposed 4x4 in registers (v_perm_b32), split into planes and written to
-// a double-buffered LDS ring in [element][time] rows, so each MFMA operand
-// is one 16-byte LDS read per lane.
+// double-buffered LDS planes of 16-sample [element] slots (bank-conflict-free
+// for both the 8-byte writes and the 16-byte reads), so each MFMA operand is
+// one 16-byte LDS read per lane.
 #include "chx_internal.h"
 
 namespace chx {
@@ -42,11 +43,14 @@ constexpr int kChunk = 32768;    // samples accumulated before a flush
 #endif
 constexpr int kRing = CHX_MFMA_RING;  // stages of raw input in flight per thread
 constexpr int kRows = 2 * kTile; // LDS rows per plane: row tile, then column tile
-// LDS row of element row e: one spare 32-byte row after every 8 keeps both the
-// 8-byte transposed writes and the 16-byte operand reads free of bank conflicts.
-constexpr int kPlaneBytes = (kRows + kRows / 8) * 32;
+// LDS plane layout (64 banks x 4 B on gfx950): [k-half][row][16 B], so the
+// 16 lanes of an operand read cover 256 contiguous bytes; rows are swizzled
+// r ^ ((r >> 3) & 3) and the second half is offset by 128 B mod 256, so the
+// transposed 8-byte writes of 32 lanes land on 32 distinct bank pairs too.
+constexpr int kHalfBytes = kRows * 16 + 128;
+constexpr int kPlaneBytes = 2 * kHalfBytes;
 
-__device__ __forceinline__ int lds_row(int e) { return (e + (e >> 3)) * 32; }
+__device__ __forceinline__ int lds_slot(int r) { return (r ^ ((r >> 3) & 3)) * 16; }
 
 // 4x4 byte transpose: r[s] holds byte e of sample s; c[e] gets byte s of
 // element e.  v_perm_b32 selects bytes of {s0:s1} (selector 0-3 = s1,
@@ -110,6 +114,24 @@ __device__ __forceinline__ void flush_block(int32_t *blk, const v16i &re, const 
   }
 }
 
+// Scheduling template for one stage: the 8 operand reads first, then each
+// MFMA followed by a slice of the next stage's VALU, LDS-write and VMEM work.
+__device__ __forceinline__ void interleave_hints() {
+  __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);  // DS read
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+    __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // VALU
+    __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // DS write
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+    __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);  // VALU
+    __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
+  }
+}
+
 // grid.x = channels x upper-triangle tiles (XCD-remapped), grid.y = time
 // splits.  mode: 0 assign, 1 add (applies to the first flush; later flushes
 // of the same workgroup add); splits > 1 always add atomically.
@@ -152,12 +174,12 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(
 
   // Compute role: wave (wr, wc) = rows 64*wr.., columns 64*wc.. of the tile.
   const int wr = wave >> 1, wc = wave & 1;
-  const int kb = (lane >> 5) * 16;
+  const int kb = (lane >> 5) * kHalfBytes;
   int aoff[2], boff[2];
 #pragma unroll
   for (int i = 0; i < 2; i++) {
-    aoff[i] = lds_row(64 * wr + 32 * i + (lane & 31)) + kb;
-    boff[i] = lds_row(kTile + 64 * wc + 32 * i + (lane & 31)) + kb;
+    aoff[i] = lds_slot(64 * wr + 32 * i + (lane & 31)) + kb;
+    boff[i] = lds_slot(kTile + 64 * wc + 32 * i + (lane & 31)) + kb;
   }
 
   v16i acc_re[2][2], acc_ix[2][2], acc_iy[2][2];
@@ -192,7 +214,7 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(
       uint32_t rl, il, rh, ih;
       planes(lo[e], rl, il);
       planes(hi[e], rh, ih);
-      const int off = lds_row(lrow + e) + 8 * lg;
+      const int off = (lg >> 1) * kHalfBytes + lds_slot(lrow + e) + 8 * (lg & 1);
       *(uint2 *)&lds[buf][0][off] = make_uint2(rl, rh);
       *(uint2 *)&lds[buf][1][off] = make_uint2(il, ih);
     }
@@ -239,10 +261,15 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(
       for (int d = 0; d < D; d++) {
         const int64_t st = st0 + d;
         const int buf = (int)(st & 1);
+        // One basic block per stage: the operand reads, the 16 MFMAs, the
+        // transposition of stage st+1 into the other LDS buffer (harmless
+        // after the last stage) and the loads of stage st+1+D, interleaved so
+        // the VALU/LDS/VMEM work issues while the matrix core is busy.
         compute_stage(buf);
-        if (st + 1 < nst) store_stage(ring[(d + 1) % D], buf ^ 1);
+        store_stage(ring[(d + 1) % D], buf ^ 1);
         const int64_t nxt = st + 1 + D;
         load_stage(ring[(d + 1) % D], nxt < last ? nxt : last);
+        interleave_hints();
         __syncthreads();
       }
     }
