@@ -165,8 +165,13 @@ int chx_plan_create(const chx_config *cfg, chx_plan **out) {
   if (e == hipSuccess)
     e = hipMemcpy(p->d_maps, maps, sizeof(uint32_t) * 2 * p->NB, hipMemcpyHostToDevice);
   free(maps);
+  p->ntiles = (cfg->num_elements + 127) / 128;
   if (e == hipSuccess && cfg->kernel_batch != CHX_METHOD_INT8_MFMA)
     e = hipMalloc((void **)&p->d_sums, p->sums_bytes);
+  if (e == hipSuccess && cfg->kernel_batch == CHX_METHOD_INT8_MFMA) {
+    p->work_bytes = mfma_workspace_bytes(p);
+    e = hipMalloc((void **)&p->d_work, p->work_bytes);
+  }
   if (e != hipSuccess) {
     rc = hip_status(e);
     chx_plan_destroy(p);
@@ -181,6 +186,7 @@ int chx_plan_destroy(chx_plan *p) {
   (void)hipSetDevice(p->cfg.device);
   if (p->d_maps) (void)hipFree(p->d_maps);
   if (p->d_sums) (void)hipFree(p->d_sums);
+  if (p->d_work) (void)hipFree(p->d_work);
   if (p->d_in) (void)hipFree(p->d_in);
   if (p->d_out) (void)hipFree(p->d_out);
   free(p);

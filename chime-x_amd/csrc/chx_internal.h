@@ -12,6 +12,9 @@ struct chx_plan {
   int64_t NB;        // upper-triangle block count
   uint32_t *d_maps;  // [2][NB]: id_x_map then id_y_map (main_wrapper.c:528-552)
   uint32_t *d_sums;  // [F][N][2] plan-owned offset sums (methods 0/1)
+  int32_t ntiles;    // 128-element tiles per side (int8 matrix-core path)
+  uint8_t *d_work;   // int8 planes of one time chunk (int8 matrix-core path)
+  size_t work_bytes;
   // Lazily allocated buffers of chx_correlate_host.
   uint8_t *d_in;
   int32_t *d_out;
@@ -43,6 +46,9 @@ int launch_blocks_to_triangle(int32_t F, int32_t N, const int32_t *blocks,
                               int32_t *tri, hipStream_t s);
 
 // int8 matrix-core path (chx_mfma.hip).  mode: 0 assign, 1 add.
+// The plane workspace of one time chunk is at most kWorkspaceBytes.
+constexpr int64_t kWorkspaceBytes = 4LL << 30;
+size_t mfma_workspace_bytes(const chx_plan *p);
 int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis,
                      int mode, hipStream_t s);
 

@@ -11,21 +11,20 @@
 //        IY - IX for 0 (cpu_corr_test.c:92).
 // No offset/preseed correction is needed and there is no time_accum cap.
 //
-// Operand encoding: a byte with nibbles (R, I) is XORed with 0x88 so each
-// nibble holds its value's 4-bit two's complement; masking a nibble into
-// the high half of an int8 gives 16*value exactly.  Every product is then
-// 256x the true one, accumulated in int32 for at most kChunk samples
-// (|RE| <= 256*128*kChunk = 2^30) and divided by 256 exactly when the chunk
-// is flushed to the int32 output.
-//
-// Tiling: workgroup = 4 wavefronts = a 128x128-element output tile of one
-// channel (16 reference blocks), wavefront = 64x64 (2x2 blocks, 3 int32
-// accumulators of 16 registers each per block), v_mfma_i32_32x32x32_i8 with
-// K = 32 samples per stage.  Stages are loaded as [time][element] bytes,
-// transposed 4x4 in registers (v_perm_b32), split into planes and written to
-// double-buffered LDS planes of 16-sample [element] slots (bank-conflict-free
-// for both the 8-byte writes and the 16-byte reads), so each MFMA operand is
-// one 16-byte LDS read per lane.
+// Two passes per chunk of time:
+//  1. unpack_planes_kernel (HBM-bound): [time][freq][element] bytes ->
+//     MFMA-ready planes [plane][freq][16-sample block][element][16 B].  A
+//     byte with nibbles (R, I) is XORed with 0x88 so each nibble holds its
+//     value's 4-bit two's complement; masked into the high half of an int8
+//     it is 16*value exactly.  Elements past N and samples past T are 0.
+//  2. corr_mfma_kernel (matrix-core-bound): a workgroup = 4 wavefronts = a
+//     128x128-element tile of one channel (16 reference blocks), each wave a
+//     64x64 quarter (2x2 blocks x 3 int32 accumulators), fed by buffer loads
+//     that go straight to LDS (no VGPR staging, no VALU) through a ring of
+//     4 stages of 64 samples; waves only read operands and issue MFMAs.
+// Every product is 256x the true one; accumulators hold at most kChunk
+// samples (|RE| <= 256*128*kChunk = 2^30) and are divided by 256 (exact)
+// when flushed to the int32 output.
 #include "chx_internal.h"
 
 namespace chx {
@@ -34,23 +33,19 @@ namespace {
 typedef int v2i __attribute__((ext_vector_type(2)));
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) void lds_void;
 
-constexpr int kTile = 128;       // elements per tile side
-constexpr int kK = 32;           // samples per stage (MFMA K)
-constexpr int kChunk = 32768;    // samples accumulated before a flush
-#ifndef CHX_MFMA_RING
-#define CHX_MFMA_RING 4
-#endif
-constexpr int kRing = CHX_MFMA_RING;  // stages of raw input in flight per thread
-constexpr int kRows = 2 * kTile; // LDS rows per plane: row tile, then column tile
-// LDS plane layout (64 banks x 4 B on gfx950): [k-half][row][16 B], so the
-// 16 lanes of an operand read cover 256 contiguous bytes; rows are swizzled
-// r ^ ((r >> 3) & 3) and the second half is offset by 128 B mod 256, so the
-// transposed 8-byte writes of 32 lanes land on 32 distinct bank pairs too.
-constexpr int kHalfBytes = kRows * 16 + 128;
-constexpr int kPlaneBytes = 2 * kHalfBytes;
-
-__device__ __forceinline__ int lds_slot(int r) { return (r ^ ((r >> 3) & 3)) * 16; }
+constexpr int kTile = 128;     // elements per tile side
+constexpr int kStage = 64;     // samples per LDS stage (2 MFMA k-steps)
+constexpr int kRing = 4;       // LDS stages in flight
+constexpr int kChunk = 32768;  // samples accumulated before a flush
+// LDS stage layout: [plane 2][row group 8][16-sample block 4][32 rows][16 B].
+// A 32-row, 32-sample operand is 1 KB contiguous: lanes 0-31 read block
+// 2k, lanes 32-63 block 2k+1, 16 lanes = 256 contiguous bytes (no bank
+// conflicts); one 64-lane direct load fills exactly one operand.
+constexpr int kGroupBytes = 4 * 512;
+constexpr int kStageBytes = 2 * 8 * kGroupBytes;  // 32 KB
 
 // 4x4 byte transpose: r[s] holds byte e of sample s; c[e] gets byte s of
 // element e.  v_perm_b32 selects bytes of {s0:s1} (selector 0-3 = s1,
@@ -114,41 +109,69 @@ __device__ __forceinline__ void flush_block(int32_t *blk, const v16i &re, const 
   }
 }
 
-// Scheduling template for one stage: the 8 operand reads first, then each
-// MFMA followed by a slice of the next stage's VALU, LDS-write and VMEM work.
-__device__ __forceinline__ void interleave_hints() {
-  __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);  // DS read
+// Pass 1.  Workgroup = 256 element quads (1024 elements) x 16 samples of
+// one channel.  Thread: 16 word loads (one per sample row; 256 contiguous
+// bytes per wave and row), four 4x4 byte transposes, plane split; the
+// planes go through LDS so that every global store writes 1 KB of
+// consecutive 16-byte element chunks (full cache lines).
+__global__ __launch_bounds__(256) void unpack_planes_kernel(
+    const uint8_t *__restrict__ in, uint8_t *__restrict__ out_planes, int N, int F, int Np,
+    int64_t row_bytes, int64_t t0, int64_t t_valid, int KB) {
+  __shared__ __attribute__((aligned(16))) v4u stage[2][1024 + 64];  // [plane][element], padded
+  const int tid = threadIdx.x;
+  const int q = blockIdx.x * 256 + tid;  // element quad
+  const int kb = blockIdx.y, f = blockIdx.z;
+  const bool evalid = 4 * q < N;
+  uint32_t w[16];
+  const uint8_t *src = in + (t0 + 16 * (int64_t)kb) * row_bytes + (int64_t)f * N + 4 * q;
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
-    __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // VALU
-    __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // DS write
+  for (int s = 0; s < 16; s++) {
+    const bool ok = evalid && (16 * (int64_t)kb + s < t_valid);
+    w[s] = ok ? *(const uint32_t *)(src + s * row_bytes) : 0x88888888u;
   }
+  uint32_t c[4][4];  // [element][4-sample group]
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
-    __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);  // VALU
-    __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read
+  for (int m = 0; m < 4; m++)
+    transpose4(w[4 * m], w[4 * m + 1], w[4 * m + 2], w[4 * m + 3], c[0][m], c[1][m], c[2][m],
+               c[3][m]);
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    uint32_t r[4], i[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) planes(c[e][m], r[m], i[m]);
+    const int el = 4 * tid + e;
+    const int slot = el + (el >> 4);  // one pad chunk per 16: conflict-free b128 writes
+    stage[0][slot] = (v4u){r[0], r[1], r[2], r[3]};
+    stage[1][slot] = (v4u){i[0], i[1], i[2], i[3]};
+  }
+  __syncthreads();
+  const int64_t plane_stride = (int64_t)F * KB * Np * 16;
+  const int e_base = blockIdx.x * 1024;
+  uint8_t *dst = out_planes + (((int64_t)f * KB + kb) * Np + e_base) * 16;
+  const int n_el = Np - e_base < 1024 ? Np - e_base : 1024;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int el = k * 256 + tid;
+    if (el < n_el) {
+      const int slot = el + (el >> 4);
+      *(v4u *)(dst + el * 16) = stage[0][slot];
+      *(v4u *)(dst + plane_stride + el * 16) = stage[1][slot];
+    }
   }
 }
 
-// grid.x = channels x upper-triangle tiles (XCD-remapped), grid.y = time
-// splits.  mode: 0 assign, 1 add (applies to the first flush; later flushes
-// of the same workgroup add); splits > 1 always add atomically.
-//
-// Pipeline: the raw words of stage s sit in register ring slot s % D, loaded
-// D stages before they are transposed into LDS, so each load has D stages of
-// matrix work (D * 512 cycles) to arrive; LDS planes are double-buffered, one
-// barrier per stage.
-template <int D>
+// Pass 2.  grid.x = channels x upper-triangle tiles (XCD-remapped),
+// grid.y = time splits of the chunk (each a whole number of stages).
+// mode: 0 assign, 1 add (first flush; later flushes add), 2 atomic add.
 __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(
-    const uint8_t *__restrict__ in, int32_t *__restrict__ vis, int N, int F,
-    int64_t row_bytes, int64_t t_split, int ntiles, int64_t NB, int nb, int U,
-    int mode) {
-  static_assert((kChunk / kK) % D == 0, "ring depth");
-  __shared__ __attribute__((aligned(16))) uint8_t lds[2][2][kPlaneBytes];  // [buf][re/im]
+    const uint8_t *__restrict__ planes, int32_t *__restrict__ vis, int F, int Np, int KB,
+    int stages_per_split, int ntiles, int64_t NB, int nb, int U, int mode) {
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kStageBytes];
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // wave index made provably uniform: descriptors and LDS destinations built
+  // from it stay in scalar registers.
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int64_t total = (int64_t)gridDim.x;
   const int64_t item = xcd_remap(blockIdx.x, total);
   const int64_t per_f = (int64_t)ntiles * (ntiles + 1) / 2;
@@ -160,117 +183,97 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(
     ti++;
   }
   const int tj = ti + tb;  // tile column >= tile row
-  const int64_t t_begin = (int64_t)blockIdx.y * t_split;
+  const int st_begin = blockIdx.y * stages_per_split;
+  const int nst = stages_per_split;
 
-  // Loader role: wave w loads quads 16w..16w+15 (quads 0-31: row tile,
-  // 32-63: column tile), lane = (quad, 8-sample group).  Quads beyond N read
-  // element 0 of the same row (always in bounds) and are replaced by the
-  // byte 0x88, the value (0, 0).
-  const int lq = 16 * wave + (lane >> 2), lg = lane & 3;
-  const int le = (lq & 31) * 4;                       // element within its tile
-  const int lelem = (lq < 32 ? ti : tj) * kTile + le; // absolute element
-  const bool lvalid = lelem < N;
-  const int lrow = (lq < 32 ? 0 : kTile) + le;        // first LDS row of the quad
+  // Loads: wave w fills the 8 operands (plane w/2, row groups 4(w&1)..+3,
+  // k-steps 0/1) of each stage.  One descriptor per (plane, channel); the
+  // lane offset picks the element and the 16-sample block parity, the
+  // scalar offset the stage's block.
+  const int lp = wave >> 1;
+  const int64_t blk_bytes = (int64_t)Np * 16;  // one 16-sample block of a channel
+  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(planes + ((int64_t)lp * F + f) * KB * blk_bytes), (short)0,
+      (int)(KB * blk_bytes), 0x00020000);
+  int voff[4];
+#pragma unroll
+  for (int gg = 0; gg < 4; gg++) {
+    const int g = 4 * (wave & 1) + gg;
+    const int e0 = (g < 4 ? ti : tj) * kTile + 32 * (g & 3);
+    voff[gg] = (e0 + (lane & 31)) * 16 + (lane >> 5) * (int)blk_bytes;
+  }
+  auto issue_stage = [&](int st, int slot) {
+    const int kb0 = 4 * (st_begin + st);
+#pragma unroll
+    for (int gg = 0; gg < 4; gg++)
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        const int g = 4 * (wave & 1) + gg;
+        lds_void *dst = (lds_void *)&lds[slot][((lp * 8 + g) * 4 + 2 * k) * 512];
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, dst, 16, voff[gg],
+                                                 (kb0 + 2 * k) * (int)blk_bytes, 0, 0);
+      }
+  };
 
   // Compute role: wave (wr, wc) = rows 64*wr.., columns 64*wc.. of the tile.
   const int wr = wave >> 1, wc = wave & 1;
-  const int kb = (lane >> 5) * kHalfBytes;
-  int aoff[2], boff[2];
-#pragma unroll
-  for (int i = 0; i < 2; i++) {
-    aoff[i] = lds_slot(64 * wr + 32 * i + (lane & 31)) + kb;
-    boff[i] = lds_slot(kTile + 64 * wc + 32 * i + (lane & 31)) + kb;
-  }
-
   v16i acc_re[2][2], acc_ix[2][2], acc_iy[2][2];
   const v16i zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc_re[i][j] = acc_ix[i][j] = acc_iy[i][j] = zero;
-
-  uint32_t ring[D][8];
-  // Buffer loads: the descriptor is rebased on the stage's first row (a
-  // scalar), the lane offset is one VGPR and the sample row is the scalar
-  // offset, so the eight loads of a stage need no 64-bit address math.
-  const uint8_t *chan = in + t_begin * row_bytes + (int64_t)f * N;
-  const int voff = 8 * lg * (int)row_bytes + (lvalid ? lelem : 0);
-  const int rows_bytes = (int)(kK * row_bytes);
-  auto load_stage = [&](uint32_t (&w)[8], int64_t st) {
-    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)(chan + st * kK * row_bytes), (short)0, rows_bytes, 0x00020000);
+  auto compute_stage = [&](int slot) {
 #pragma unroll
-    for (int s = 0; s < 8; s++)
-      w[s] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, s * (int)row_bytes, 0);
-  };
-  auto store_stage = [&](uint32_t (&w)[8], int buf) {
-    uint32_t lo[4], hi[4];
+    for (int k = 0; k < 2; k++) {
+      v4i ar[2], ai[2], br[2], bi[2];
 #pragma unroll
-    for (int s = 0; s < 8; s++) w[s] = lvalid ? w[s] : 0x88888888u;
-    transpose4(w[0], w[1], w[2], w[3], lo[0], lo[1], lo[2], lo[3]);
-    transpose4(w[4], w[5], w[6], w[7], hi[0], hi[1], hi[2], hi[3]);
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-      uint32_t rl, il, rh, ih;
-      planes(lo[e], rl, il);
-      planes(hi[e], rh, ih);
-      const int off = (lg >> 1) * kHalfBytes + lds_slot(lrow + e) + 8 * (lg & 1);
-      *(uint2 *)&lds[buf][0][off] = make_uint2(rl, rh);
-      *(uint2 *)&lds[buf][1][off] = make_uint2(il, ih);
-    }
-  };
-  auto compute_stage = [&](int buf) {
-    v4i ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-    for (int i = 0; i < 2; i++) {
-      ar[i] = *(const v4i *)&lds[buf][0][aoff[i]];
-      ai[i] = *(const v4i *)&lds[buf][1][aoff[i]];
-      br[i] = *(const v4i *)&lds[buf][0][boff[i]];
-      bi[i] = *(const v4i *)&lds[buf][1][boff[i]];
-    }
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-      for (int j = 0; j < 2; j++) {
-        acc_re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[i], br[j], acc_re[i][j], 0, 0, 0);
-        acc_re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ai[i], bi[j], acc_re[i][j], 0, 0, 0);
-        acc_ix[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ai[i], br[j], acc_ix[i][j], 0, 0, 0);
-        acc_iy[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[i], bi[j], acc_iy[i][j], 0, 0, 0);
+      for (int i = 0; i < 2; i++) {
+        const int ga = 2 * wr + i, gb = 4 + 2 * wc + i;
+        ar[i] = *(const v4i *)&lds[slot][((0 * 8 + ga) * 4 + 2 * k) * 512 + lane * 16];
+        ai[i] = *(const v4i *)&lds[slot][((1 * 8 + ga) * 4 + 2 * k) * 512 + lane * 16];
+        br[i] = *(const v4i *)&lds[slot][((0 * 8 + gb) * 4 + 2 * k) * 512 + lane * 16];
+        bi[i] = *(const v4i *)&lds[slot][((1 * 8 + gb) * 4 + 2 * k) * 512 + lane * 16];
       }
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          acc_re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[i], br[j], acc_re[i][j], 0, 0, 0);
+          acc_iy[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[i], bi[j], acc_iy[i][j], 0, 0, 0);
+          acc_ix[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ai[i], br[j], acc_ix[i][j], 0, 0, 0);
+          acc_re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ai[i], bi[j], acc_re[i][j], 0, 0, 0);
+        }
+    }
   };
-
-  const int64_t nst = t_split / kK;
   int cur_mode = mode;
-  // Output blocks of this wave (upper triangle, inside N).
   const int by0 = (ti * kTile + 64 * wr) / 32, bx0 = (tj * kTile + 64 * wc) / 32;
 
-  // Every load is issued unconditionally (stage index clamped into range)
-  // so the wait counts stay exact: stage st+1's words are the oldest D*8
-  // loads in flight when they are consumed.
-  const int64_t last = nst - 1;
+  // Prologue: stages 0..kRing-2 in flight; wait for stage 0 (the 8 loads of
+  // each later stage may still be pending).
 #pragma unroll
-  for (int d = 0; d < D; d++) load_stage(ring[d], d < last ? d : last);
-  store_stage(ring[0], 0);
-  load_stage(ring[0], D < last ? D : last);
-  __syncthreads();
+  for (int s = 0; s < kRing - 1; s++) issue_stage(s < nst ? s : nst - 1, s);
+  asm volatile("s_waitcnt vmcnt(16)\n\ts_barrier" ::: "memory");
 
-  for (int64_t c0 = 0; c0 < nst; c0 += kChunk / kK) {
-    const int64_t c1 = c0 + kChunk / kK < nst ? c0 + kChunk / kK : nst;
-    for (int64_t st0 = c0; st0 < c1; st0 += D) {
+  const int stages_per_chunk = kChunk / kStage;
+  for (int c0 = 0; c0 < nst; c0 += stages_per_chunk) {
+    const int c1 = c0 + stages_per_chunk < nst ? c0 + stages_per_chunk : nst;
+    for (int st0 = c0; st0 < c1; st0 += kRing) {
 #pragma unroll
-      for (int d = 0; d < D; d++) {
-        const int64_t st = st0 + d;
-        const int buf = (int)(st & 1);
-        // One basic block per stage: the operand reads, the 16 MFMAs, the
-        // transposition of stage st+1 into the other LDS buffer (harmless
-        // after the last stage) and the loads of stage st+1+D, interleaved so
-        // the VALU/LDS/VMEM work issues while the matrix core is busy.
-        compute_stage(buf);
-        store_stage(ring[(d + 1) % D], buf ^ 1);
-        const int64_t nxt = st + 1 + D;
-        load_stage(ring[(d + 1) % D], nxt < last ? nxt : last);
-        interleave_hints();
-        __syncthreads();
+      for (int d = 0; d < kRing; d++) {
+        const int st = st0 + d;
+        if (st < c1) {
+          // Refill the slot read in the previous stage (every wave passed the
+          // barrier after reading it); stages past the end reload the last
+          // stage, whose data is never used.
+          const int nx = st + kRing - 1;
+          issue_stage(nx < nst ? nx : nst - 1, (d + kRing - 1) % kRing);
+          compute_stage(d);
+          // Stage st+1's loads (this wave's) have landed once at most the 16
+          // loads of the two younger stages are pending; the barrier makes
+          // every wave's part visible.
+          asm volatile("s_waitcnt vmcnt(16)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        }
       }
     }
 #pragma unroll
@@ -287,46 +290,66 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(
       }
     if (cur_mode == 0) cur_mode = 1;
   }
+  // Drain the trailing (unused) loads before the waves exit.
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 }  // namespace
 
+// Samples per time chunk so that the plane workspace stays within
+// kWorkspaceBytes (a multiple of the stage length).
+static int64_t chunk_samples(const chx_plan *p) {
+  const int64_t Np = (int64_t)p->ntiles * kTile;
+  const int64_t per_sample = 2 * (int64_t)p->cfg.num_freq * Np;
+  int64_t tc = kWorkspaceBytes / per_sample;
+  // one (plane, channel) region must stay addressable by a buffer descriptor
+  if (tc > 0x7FFFFFFFLL / Np) tc = 0x7FFFFFFFLL / Np;
+  tc = tc / kStage * kStage;
+  const int64_t T = (p->cfg.time_steps + kStage - 1) / kStage * kStage;
+  if (tc > T) tc = T;
+  if (tc < kStage) tc = kStage;
+  return tc;
+}
+
+size_t mfma_workspace_bytes(const chx_plan *p) {
+  const int64_t Np = (int64_t)p->ntiles * kTile;
+  return (size_t)(2 * (int64_t)p->cfg.num_freq * Np * chunk_samples(p));
+}
+
 int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode,
                      hipStream_t s) {
-  const int N = p->cfg.num_elements, F = p->cfg.num_freq;
+  const int N = p->cfg.num_elements, F = p->cfg.num_freq, U = p->cfg.upper_triangle_convention;
   const int64_t T = p->cfg.time_steps;
-  const int ntiles = (N + kTile - 1) / kTile;
+  const int ntiles = p->ntiles, Np = ntiles * kTile;
   const int64_t items = (int64_t)F * ntiles * (ntiles + 1) / 2;
-  // Split time only when the tiles alone cannot fill 256 CUs twice over;
-  // keep every split a whole number of stages.
-  const int64_t stages = T / kK;
-  int64_t split = (512 + items - 1) / items;
-  if (split > stages / 32) split = stages / 32;  // >= 1024 samples per split
-  if (split < 1) split = 1;
-  while (stages % split) split--;
-  const int64_t t_split = T / split;
-  int kmode = mode;
-  if (split > 1) {
-    if (mode == 0) {
-      int rc = launch_fill_u32((uint32_t *)vis, 0u, (int64_t)F * p->NB * 2048, s);
-      if (rc) return rc;
+  if (items > 0x7FFFFFFF || !p->d_work) return CHX_ERR_SHAPE;
+  const int64_t tc = chunk_samples(p);
+  const int64_t row_bytes = (int64_t)F * N;
+  for (int64_t t0 = 0; t0 < T; t0 += tc) {
+    const int64_t tv = T - t0 < tc ? T - t0 : tc;          // valid samples
+    const int nst_chunk = (int)((tv + kStage - 1) / kStage);  // stages
+    const int KB = nst_chunk * (kStage / 16);
+    unpack_planes_kernel<<<dim3((unsigned)((Np + 1023) / 1024), (unsigned)KB, (unsigned)F),
+                           dim3(256), 0, s>>>(in, p->d_work, N, F, Np, row_bytes, t0, tv, KB);
+    CHX_HIP(hipGetLastError());
+    // Split time only when the tiles alone cannot fill the chip twice over,
+    // keeping >= 16 stages per split.
+    int split = (int)((512 + items - 1) / items);
+    if (split > nst_chunk / 16) split = nst_chunk / 16;
+    if (split < 1) split = 1;
+    while (nst_chunk % split) split--;
+    int kmode = (t0 == 0) ? mode : 1;
+    if (split > 1) {
+      if (kmode == 0) {
+        int rc = launch_fill_u32((uint32_t *)vis, 0u, (int64_t)F * p->NB * 2048, s);
+        if (rc) return rc;
+      }
+      kmode = 2;
     }
-    kmode = 2;
+    corr_mfma_kernel<<<dim3((unsigned)items, (unsigned)split), dim3(256), 0, s>>>(
+        p->d_work, vis, F, Np, KB, nst_chunk / split, ntiles, p->NB, p->nb, U, kmode);
+    CHX_HIP(hipGetLastError());
   }
-  if (items > 0x7FFFFFFF) return CHX_ERR_SHAPE;
-  const dim3 grid((unsigned)items, (unsigned)split), blk(256);
-  const int64_t nst = t_split / kK;
-  const int U = p->cfg.upper_triangle_convention;
-  const int64_t rb = (int64_t)F * N;
-  // The stage loop runs in groups of the ring depth: pick the deepest ring
-  // that divides the per-split stage count.
-  if (nst % kRing == 0)
-    corr_mfma_kernel<kRing><<<grid, blk, 0, s>>>(in, vis, N, F, rb, t_split, ntiles, p->NB, p->nb, U, kmode);
-  else if (nst % 2 == 0)
-    corr_mfma_kernel<2><<<grid, blk, 0, s>>>(in, vis, N, F, rb, t_split, ntiles, p->NB, p->nb, U, kmode);
-  else
-    corr_mfma_kernel<1><<<grid, blk, 0, s>>>(in, vis, N, F, rb, t_split, ntiles, p->NB, p->nb, U, kmode);
-  CHX_HIP(hipGetLastError());
   return CHX_SUCCESS;
 }
 
