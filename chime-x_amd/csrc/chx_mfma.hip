@@ -82,8 +82,9 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t id, int64_t total) {
 // Flush one block's accumulators.  Lane l owns column n = l%32 and rows
 // m = 8*(j/4) + 4*(l/32) + j%4 of register j (v_mfma_i32_32x32x32_i8 D
 // layout).  mode 0 assigns, 1 adds, 2 adds atomically.
-__device__ __forceinline__ void flush_block(int32_t *blk, const v16i &re, const v16i &ix,
-                                            const v16i &iy, int U, int mode, int lane) {
+__device__ __forceinline__ void flush_block(int32_t *blk, const v16i &rea, const v16i &reb,
+                                            const v16i &ix, const v16i &iy, int U, int mode,
+                                            int lane) {
   // Buffer ops on a descriptor of the block (a wave-uniform base): one VGPR
   // of lane offset, the register's row as a constant scalar offset.
   const __amdgpu_buffer_rsrc_t rsrc =
@@ -91,7 +92,7 @@ __device__ __forceinline__ void flush_block(int32_t *blk, const v16i &re, const 
   const int voff = ((lane >> 5) * 4 * 32 + (lane & 31)) * 8;
 #pragma unroll
   for (int j = 0; j < 16; j++) {
-    const int32_t vr = re[j] >> 8;
+    const int32_t vr = (rea[j] >> 8) + (reb[j] >> 8);
     const int32_t vi = U ? (ix[j] - iy[j]) >> 8 : (iy[j] - ix[j]) >> 8;
     const int soff = (8 * (j >> 2) + (j & 3)) * 32 * 8;  // row m = 8(j/4) + 4(l/32) + j%4
     if (mode == 0) {
@@ -216,45 +217,55 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(
   };
 
   // Compute role: wave (wr, wc) = rows 64*wr.., columns 64*wc.. of the tile.
+  // 16 accumulators (RE split in two), so each is updated once per 16 MFMAs.
   const int wr = wave >> 1, wc = wave & 1;
-  v16i acc_re[2][2], acc_ix[2][2], acc_iy[2][2];
+  v16i rea[2][2], reb[2][2], acc_ix[2][2], acc_iy[2][2];
   const v16i zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc_re[i][j] = acc_ix[i][j] = acc_iy[i][j] = zero;
-  auto compute_stage = [&](int slot) {
+    for (int j = 0; j < 2; j++) rea[i][j] = reb[i][j] = acc_ix[i][j] = acc_iy[i][j] = zero;
+  struct Ops {
+    v4i ar[2], ai[2], br[2], bi[2];
+  };
+  auto read_ops = [&](Ops &o, int slot, int k) {
 #pragma unroll
-    for (int k = 0; k < 2; k++) {
-      v4i ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-      for (int i = 0; i < 2; i++) {
-        const int ga = 2 * wr + i, gb = 4 + 2 * wc + i;
-        ar[i] = *(const v4i *)&lds[slot][((0 * 8 + ga) * 4 + 2 * k) * 512 + lane * 16];
-        ai[i] = *(const v4i *)&lds[slot][((1 * 8 + ga) * 4 + 2 * k) * 512 + lane * 16];
-        br[i] = *(const v4i *)&lds[slot][((0 * 8 + gb) * 4 + 2 * k) * 512 + lane * 16];
-        bi[i] = *(const v4i *)&lds[slot][((1 * 8 + gb) * 4 + 2 * k) * 512 + lane * 16];
-      }
-#pragma unroll
-      for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-          acc_re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[i], br[j], acc_re[i][j], 0, 0, 0);
-          acc_iy[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[i], bi[j], acc_iy[i][j], 0, 0, 0);
-          acc_ix[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ai[i], br[j], acc_ix[i][j], 0, 0, 0);
-          acc_re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ai[i], bi[j], acc_re[i][j], 0, 0, 0);
-        }
+    for (int i = 0; i < 2; i++) {
+      const int ga = 2 * wr + i, gb = 4 + 2 * wc + i;
+      o.ar[i] = *(const v4i *)&lds[slot][((0 * 8 + ga) * 4 + 2 * k) * 512 + lane * 16];
+      o.br[i] = *(const v4i *)&lds[slot][((0 * 8 + gb) * 4 + 2 * k) * 512 + lane * 16];
+      o.ai[i] = *(const v4i *)&lds[slot][((1 * 8 + ga) * 4 + 2 * k) * 512 + lane * 16];
+      o.bi[i] = *(const v4i *)&lds[slot][((1 * 8 + gb) * 4 + 2 * k) * 512 + lane * 16];
     }
+  };
+  auto mfma_ops = [&](const Ops &o) {
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        rea[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(o.ar[i], o.br[j], rea[i][j], 0, 0, 0);
+        acc_iy[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(o.ar[i], o.bi[j], acc_iy[i][j], 0, 0, 0);
+        acc_ix[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(o.ai[i], o.br[j], acc_ix[i][j], 0, 0, 0);
+        reb[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(o.ai[i], o.bi[j], reb[i][j], 0, 0, 0);
+      }
   };
   int cur_mode = mode;
   const int by0 = (ti * kTile + 64 * wr) / 32, bx0 = (tj * kTile + 64 * wc) / 32;
 
   // Prologue: stages 0..kRing-2 in flight; wait for stage 0 (the 8 loads of
-  // each later stage may still be pending).
+  // each later stage may still be pending), read its first operands.
 #pragma unroll
   for (int s = 0; s < kRing - 1; s++) issue_stage(s < nst ? s : nst - 1, s);
   asm volatile("s_waitcnt vmcnt(16)\n\ts_barrier" ::: "memory");
+  Ops o0, o1;
+  read_ops(o0, 0, 0);
 
+  // Per stage st (slot st % kRing; the stage count is a multiple of kRing):
+  //   refill the slot of stage st-1 with stage st+kRing-1;
+  //   read k-step 1 of st, MFMAs of k-step 0;
+  //   wait for this wave's loads of st+1, barrier (all waves' loads landed,
+  //   all reads of the refilled slot done);
+  //   read k-step 0 of st+1, MFMAs of k-step 1.
   const int stages_per_chunk = kChunk / kStage;
   for (int c0 = 0; c0 < nst; c0 += stages_per_chunk) {
     const int c1 = c0 + stages_per_chunk < nst ? c0 + stages_per_chunk : nst;
@@ -262,18 +273,13 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(
 #pragma unroll
       for (int d = 0; d < kRing; d++) {
         const int st = st0 + d;
-        if (st < c1) {
-          // Refill the slot read in the previous stage (every wave passed the
-          // barrier after reading it); stages past the end reload the last
-          // stage, whose data is never used.
-          const int nx = st + kRing - 1;
-          issue_stage(nx < nst ? nx : nst - 1, (d + kRing - 1) % kRing);
-          compute_stage(d);
-          // Stage st+1's loads (this wave's) have landed once at most the 16
-          // loads of the two younger stages are pending; the barrier makes
-          // every wave's part visible.
-          asm volatile("s_waitcnt vmcnt(16)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-        }
+        const int nx = st + kRing - 1;
+        issue_stage(nx < nst ? nx : nst - 1, (d + kRing - 1) % kRing);
+        read_ops(o1, d, 1);
+        mfma_ops(o0);
+        asm volatile("s_waitcnt vmcnt(16)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        read_ops(o0, (d + 1) % kRing, 0);
+        mfma_ops(o1);
       }
     }
 #pragma unroll
@@ -283,10 +289,10 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(
         const int by = by0 + i, bx = bx0 + j;
         if (bx >= by && bx < nb) {
           const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
-          flush_block(vis + ((int64_t)f * NB + b) * 2048, acc_re[i][j], acc_ix[i][j],
+          flush_block(vis + ((int64_t)f * NB + b) * 2048, rea[i][j], reb[i][j], acc_ix[i][j],
                       acc_iy[i][j], U, cur_mode, lane);
         }
-        acc_re[i][j] = acc_ix[i][j] = acc_iy[i][j] = zero;
+        rea[i][j] = reb[i][j] = acc_ix[i][j] = acc_iy[i][j] = zero;
       }
     if (cur_mode == 0) cur_mode = 1;
   }
@@ -304,10 +310,10 @@ static int64_t chunk_samples(const chx_plan *p) {
   int64_t tc = kWorkspaceBytes / per_sample;
   // one (plane, channel) region must stay addressable by a buffer descriptor
   if (tc > 0x7FFFFFFFLL / Np) tc = 0x7FFFFFFFLL / Np;
-  tc = tc / kStage * kStage;
-  const int64_t T = (p->cfg.time_steps + kStage - 1) / kStage * kStage;
+  tc = tc / (kStage * kRing) * (kStage * kRing);
+  const int64_t T = (p->cfg.time_steps + kStage * kRing - 1) / (kStage * kRing) * (kStage * kRing);
   if (tc > T) tc = T;
-  if (tc < kStage) tc = kStage;
+  if (tc < kStage * kRing) tc = kStage * kRing;
   return tc;
 }
 
@@ -327,7 +333,8 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
   const int64_t row_bytes = (int64_t)F * N;
   for (int64_t t0 = 0; t0 < T; t0 += tc) {
     const int64_t tv = T - t0 < tc ? T - t0 : tc;          // valid samples
-    const int nst_chunk = (int)((tv + kStage - 1) / kStage);  // stages
+    // stages, padded to a multiple of the ring (the padding samples are 0)
+    const int nst_chunk = (int)((tv + kStage * kRing - 1) / (kStage * kRing)) * kRing;
     const int KB = nst_chunk * (kStage / 16);
     unpack_planes_kernel<<<dim3((unsigned)((Np + 1023) / 1024), (unsigned)KB, (unsigned)F),
                            dim3(256), 0, s>>>(in, p->d_work, N, F, Np, row_bytes, t0, tv, KB);
@@ -337,7 +344,7 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
     int split = (int)((512 + items - 1) / items);
     if (split > nst_chunk / 16) split = nst_chunk / 16;
     if (split < 1) split = 1;
-    while (nst_chunk % split) split--;
+    while ((nst_chunk / kRing) % split) split--;  // splits hold whole rings
     int kmode = (t0 == 0) ? mode : 1;
     if (split > 1) {
       if (kmode == 0) {
