@@ -1,0 +1,128 @@
+"""Autotuner of the batch launch shape (chx_batch_tuning).
+
+For a job list it times every candidate shape -- the work-item time extent
+(max_item_stages: whole chunks, or pieces of 64..512 stages whose partial
+sums add atomically), the persistent workgroup count, and the workspace
+budget (which sets the number of rounds) -- on the GPU with the captured
+graph, and keeps the fastest.  Results are cached in a JSON file keyed by
+the job list's shapes, so a later batch of the same shapes loads its tuning
+without timing anything.
+
+    python chime-x_amd/autotune.py --out chime-x_amd/tuning.json
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import itertools
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+import torch  # noqa: E402  (before libchx: one HIP runtime in the process)
+
+import chx  # noqa: E402
+import workloads as W  # noqa: E402
+
+DEFAULT_CACHE = os.path.join(HERE, "tuning.json")
+
+
+def signature(jobs) -> str:
+    shapes = sorted((j["num_elements"], j["num_freq"], j["time_steps"],
+                     j.get("upper_triangle_convention", 1)) for j in jobs)
+    return hashlib.sha1(json.dumps(shapes).encode()).hexdigest()[:16]
+
+
+def candidates(cus: int):
+    stages = (0, 512, 256, 128, 64)
+    wgs = (cus,)
+    budgets = (16 << 30, 64 << 30)
+    for st, wg, ws in itertools.product(stages, wgs, budgets):
+        yield dict(max_item_stages=st, workgroups=wg, workspace_bytes=ws)
+
+
+def time_batch(descs, tuning, reps: int = 3) -> float:
+    """Milliseconds per graph replay (median of reps)."""
+    with chx.Batch(descs, tuning) as b:
+        s = torch.cuda.current_stream().cuda_stream
+        b.run(stream=s)  # capture + first replay
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            b.run(stream=s)
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def tune(jobs, descs, reps: int = 3, log=print) -> dict:
+    cus = torch.cuda.get_device_properties(0).multi_processor_count // 8 * 8
+    total = sum(W.cmacs(j["num_elements"], j["num_freq"], j["time_steps"]) for j in jobs)
+    table = []
+    for c in candidates(cus):
+        try:
+            ms = time_batch(descs, c, reps)
+        except chx.ChxError as e:  # e.g. a budget the device cannot hold
+            log(f"  {c}: {e}")
+            continue
+        table.append(dict(tuning=c, ms=ms, tcmacs=total / ms / 1e9))
+        log(f"  {c}: {ms:.3f} ms  {total / ms / 1e9:.1f} TCMAC/s")
+    best = min(table, key=lambda r: r["ms"])
+    return dict(signature=signature(jobs), best=best, table=table)
+
+
+def load(jobs, path: str = DEFAULT_CACHE) -> dict | None:
+    """The cached tuning of this job list's shapes, or None."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        cache = json.load(f)
+    ent = cache.get(signature(jobs))
+    return ent["best"]["tuning"] if ent else None
+
+
+def save(result: dict, path: str = DEFAULT_CACHE) -> None:
+    cache = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            cache = json.load(f)
+    cache[result["signature"]] = result
+    with open(path, "w") as f:
+        json.dump(cache, f, indent=1, sort_keys=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--njobs", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--out", default=DEFAULT_CACHE)
+    a = ap.parse_args()
+    if not torch.cuda.is_available():
+        print("autotune needs a GPU", file=sys.stderr)
+        return 2
+    jobs = W.uneven_batch_jobs(a.njobs, a.seed)
+    ins, outs, descs = [], [], []
+    for j in jobs:
+        N, F, T = j["num_elements"], j["num_freq"], j["time_steps"]
+        ins.append(W.device_uniform(torch, (T, F, N), j["seed"]))
+        outs.append(torch.empty((F, chx.num_blocks(N), 32, 32, 2), dtype=torch.int32,
+                                device="cuda"))
+        descs.append(dict(j, d_input=ins[-1].data_ptr(), d_vis=outs[-1].data_ptr()))
+    print(f"uneven batch: {len(jobs)} jobs, signature {signature(jobs)}", flush=True)
+    res = tune(jobs, descs, a.reps, log=lambda m: print(m, flush=True))
+    save(res, a.out)
+    print(json.dumps(res["best"]), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -59,6 +59,41 @@ class Config(ctypes.Structure):
     ]
 
 
+class JobDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_elements", ctypes.c_int32),
+        ("num_freq", ctypes.c_int32),
+        ("time_steps", ctypes.c_int64),
+        ("upper_triangle_convention", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("d_input", ctypes.c_void_p),
+        ("d_vis", ctypes.c_void_p),
+    ]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [
+        ("max_item_stages", ctypes.c_int32),
+        ("workgroups", ctypes.c_int32),
+        ("workspace_bytes", ctypes.c_int64),
+    ]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("num_jobs", ctypes.c_int32),
+        ("rounds", ctypes.c_int32),
+        ("chunk_samples", ctypes.c_int64),
+        ("items", ctypes.c_int64),
+        ("workgroups", ctypes.c_int32),
+        ("max_item_stages", ctypes.c_int32),
+        ("workspace_bytes", ctypes.c_size_t),
+        ("cmacs", ctypes.c_int64),
+    ]
+
+
+BATCH_EAGER = 1
+
 _lib = None
 
 
@@ -90,6 +125,13 @@ def lib() -> ctypes.CDLL:
     L.chx_max_time_accum.argtypes = [i32]
     L.chx_max_time_accum.restype = i32
     L.chx_version.restype = ctypes.c_char_p
+    L.chx_batch_create.argtypes = [i32, ctypes.POINTER(JobDesc), i32,
+                                   ctypes.POINTER(BatchTuning), ctypes.POINTER(vp)]
+    L.chx_batch_plan.argtypes = [ctypes.POINTER(JobDesc), i32, ctypes.POINTER(BatchTuning),
+                                 ctypes.POINTER(BatchInfo)]
+    L.chx_batch_run.argtypes = [vp, i32, vp]
+    L.chx_batch_get_info.argtypes = [vp, ctypes.POINTER(BatchInfo)]
+    L.chx_batch_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -206,3 +248,76 @@ def blocks_to_triangle_device(num_freq: int, N: int, d_blocks: int, d_tri: int,
                               stream: int = 0) -> None:
     _check(lib().chx_blocks_to_triangle_device(num_freq, N, d_blocks, d_tri, stream or None),
            "chx_blocks_to_triangle_device")
+
+
+def _job_array(jobs):
+    arr = (JobDesc * len(jobs))()
+    for k, j in enumerate(jobs):
+        arr[k] = JobDesc(j["num_elements"], j["num_freq"], j["time_steps"],
+                         j.get("upper_triangle_convention", 1), 0,
+                         j["d_input"], j["d_vis"])
+    return arr
+
+
+def _tuning(tuning):
+    if not tuning:
+        return None
+    return BatchTuning(tuning.get("max_item_stages", 0), tuning.get("workgroups", 0),
+                       tuning.get("workspace_bytes", 0))
+
+
+def batch_plan(jobs, tuning: dict | None = None) -> dict:
+    """Host-only planning of a batch (chx_batch_plan): rounds, items and
+    workspace, without a GPU.  Device pointers are only checked for NULL."""
+    arr = _job_array(jobs)
+    t = _tuning(tuning)
+    inf = BatchInfo()
+    _check(lib().chx_batch_plan(arr, len(jobs), ctypes.byref(t) if t else None,
+                                ctypes.byref(inf)), "chx_batch_plan")
+    return {name: getattr(inf, name) for name, _ in BatchInfo._fields_}
+
+
+class Batch:
+    """Many int8 matrix-core correlations in one captured graph (chx_batch).
+
+    jobs: sequence of dicts with num_elements, num_freq, time_steps,
+    upper_triangle_convention, d_input, d_vis (device pointers as ints).
+    tuning: None (defaults) or a dict with any of max_item_stages,
+    workgroups, workspace_bytes.
+    """
+
+    def __init__(self, jobs, tuning: dict | None = None, device: int = 0):
+        arr = _job_array(jobs)
+        self._jobs = arr
+        self.tuning = _tuning(tuning)
+        tp = ctypes.byref(self.tuning) if self.tuning else None
+        h = ctypes.c_void_p()
+        _check(lib().chx_batch_create(device, arr, len(jobs), tp, ctypes.byref(h)),
+               "chx_batch_create")
+        self._h = h
+
+    def run(self, stream: int = 0, eager: bool = False) -> None:
+        _check(lib().chx_batch_run(self._h, BATCH_EAGER if eager else 0, stream or None),
+               "chx_batch_run")
+
+    def info(self) -> dict:
+        inf = BatchInfo()
+        _check(lib().chx_batch_get_info(self._h, ctypes.byref(inf)), "chx_batch_get_info")
+        return {name: getattr(inf, name) for name, _ in BatchInfo._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().chx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

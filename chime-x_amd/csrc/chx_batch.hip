@@ -1,0 +1,439 @@
+// chx_batch.hip -- many int8 matrix-core correlations of uneven shape
+// (elements, channels, integration length) in one captured hipGraph.
+//
+// Plan (host, at creation):
+//  * time is cut into rounds of chunk_samples samples, the same for every
+//    job, sized so that one round's int8 planes of all jobs still active fit
+//    the workspace budget; a job takes part in the rounds its length reaches;
+//  * per round: one batched unpack launch over every active job's chunk, then
+//    one persistent GEMM launch over work items (job, channel, 128x128 tile,
+//    stage range); a job's tiles are cut in time when max_item_stages asks,
+//    in which case their partial sums add atomically;
+//  * items are grouped per (job, channel) and the groups spread over 8
+//    per-XCD queues by longest-processing-time first, so a channel's tiles
+//    share one XCD's L2; inside a queue, long items run first.
+// Run: counters and atomically-filled outputs are cleared with fill kernels
+// (no memset nodes), then the rounds' launches, all on one stream.
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "chx_internal.h"
+
+struct chx_batch {
+  int device = 0;
+  std::vector<chx_job_desc> jobs;
+  chx_batch_tuning tune{};
+  int64_t chunk = 0;
+  int rounds = 0;
+  int64_t nitems = 0, cmacs = 0;
+  struct Round {
+    int ujob0 = 0, nujobs = 0;
+    int64_t units = 0;
+  };
+  std::vector<Round> round;
+  struct Fill {
+    int32_t *p;
+    int64_t n;
+  };
+  std::vector<Fill> zero;  // outputs cleared before the first round
+  uint8_t *d_work = nullptr;
+  size_t work_bytes = 0;
+  chx::GemmJob *d_gjobs = nullptr;
+  chx::GemmItem *d_items = nullptr;
+  chx::UnpackJob *d_ujobs = nullptr;
+  int *d_qbound = nullptr;    // [rounds][9]
+  int *d_counters = nullptr;  // [rounds][8]
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace chx {
+namespace {
+
+constexpr int64_t kRoundSamples = (int64_t)kMfmaStage * kMfmaRing;  // 256
+constexpr int64_t kDefaultBudget = 16LL << 30;
+
+int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+int validate_job(const chx_job_desc &j) {
+  if (!j.d_input || !j.d_vis || j.reserved != 0) return CHX_ERR_INVALID_ARGUMENT;
+  if (j.upper_triangle_convention != 0 && j.upper_triangle_convention != 1)
+    return CHX_ERR_INVALID_ARGUMENT;
+  if (j.num_elements < 32 || j.num_elements % 32 != 0) return CHX_ERR_SHAPE;
+  if (j.num_freq < 1 || j.time_steps < 1) return CHX_ERR_SHAPE;
+  const int64_t nb = j.num_elements / 32, nt = (j.num_elements + kMfmaTile - 1) / kMfmaTile;
+  if ((int64_t)j.num_freq * nb * (nb + 1) / 2 > 0x7FFFFFFFLL) return CHX_ERR_SHAPE;
+  if ((int64_t)j.num_freq * nt * (nt + 1) / 2 > 0x7FFFFFFFLL) return CHX_ERR_SHAPE;
+  if (j.time_steps > (INT64_MAX / 2) / ((int64_t)j.num_freq * j.num_elements))
+    return CHX_ERR_SHAPE;
+  return CHX_SUCCESS;
+}
+
+void free_batch(chx_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->cap) (void)hipStreamDestroy(b->cap);
+  if (b->d_work) (void)hipFree(b->d_work);
+  if (b->d_gjobs) (void)hipFree(b->d_gjobs);
+  if (b->d_items) (void)hipFree(b->d_items);
+  if (b->d_ujobs) (void)hipFree(b->d_ujobs);
+  if (b->d_qbound) (void)hipFree(b->d_qbound);
+  if (b->d_counters) (void)hipFree(b->d_counters);
+  delete b;
+}
+
+template <class T>
+hipError_t upload(T **dst, const std::vector<T> &v) {
+  if (v.empty()) return hipSuccess;
+  hipError_t e = hipMalloc((void **)dst, sizeof(T) * v.size());
+  if (e == hipSuccess) e = hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+  return e;
+}
+
+// Host tables of one batch (uploaded once at creation).
+struct Tables {
+  std::vector<UnpackJob> ujobs;
+  std::vector<GemmJob> gjobs;
+  std::vector<GemmItem> items;
+  std::vector<int> qbound;
+};
+
+// Chunk length, round count, per-job workspace offsets and total size.
+int plan_sizes(chx_batch *b, std::vector<int64_t> &off) {
+  const int nj = (int)b->jobs.size();
+  std::vector<int64_t> Np(nj);
+  off.assign(nj, 0);
+  b->cmacs = 0;
+  int64_t per_sample = 0, maxT = 0;
+  for (int j = 0; j < nj; j++) {
+    const chx_job_desc &d = b->jobs[j];
+    Np[j] = round_up(d.num_elements, kMfmaTile);
+    per_sample += 2 * (int64_t)d.num_freq * Np[j];
+    maxT = std::max(maxT, d.time_steps);
+    const int64_t N = d.num_elements;
+    b->cmacs += (int64_t)d.num_freq * d.time_steps * (N * (N + 1) / 2);
+  }
+  // Chunk: as long as the budget allows (a multiple of 256 samples), no
+  // longer than the longest job, and short enough that one (plane, channel)
+  // region stays addressable by a 32-bit buffer descriptor.
+  int64_t chunk = b->tune.workspace_bytes / per_sample / kRoundSamples * kRoundSamples;
+  if (chunk < kRoundSamples) return CHX_ERR_BUDGET;
+  chunk = std::min(chunk, round_up(maxT, kRoundSamples));
+  for (int j = 0; j < nj; j++)
+    chunk = std::min<int64_t>(chunk, 0x7FFFFFFFLL / Np[j] / kRoundSamples * kRoundSamples);
+  b->chunk = chunk;
+  b->rounds = (int)((maxT + chunk - 1) / chunk);
+  int64_t ws = 0;
+  for (int j = 0; j < nj; j++) {
+    off[j] = ws;
+    ws += 2 * (int64_t)b->jobs[j].num_freq * Np[j] * std::min(chunk, round_up(b->jobs[j].time_steps, kRoundSamples));
+  }
+  b->work_bytes = (size_t)ws;
+  return CHX_SUCCESS;
+}
+
+// Every round's unpack jobs, GEMM jobs, items and queues, with plane
+// pointers into the workspace at `base`.
+int plan_tables(chx_batch *b, uint8_t *base, const std::vector<int64_t> &off, Tables &t) {
+  const int nj = (int)b->jobs.size();
+  const int64_t chunk = b->chunk;
+  std::vector<UnpackJob> &ujobs = t.ujobs;
+  std::vector<GemmJob> &gjobs = t.gjobs;
+  std::vector<GemmItem> &items = t.items;
+  std::vector<int> &qbound = t.qbound;
+  b->round.clear();
+  b->zero.clear();
+  const int max_st = b->tune.max_item_stages;  // multiple of kMfmaRing, 0 = none
+  for (int r = 0; r < b->rounds; r++) {
+    chx_batch::Round R;
+    R.ujob0 = (int)ujobs.size();
+    struct Group {
+      int64_t work;
+      std::vector<GemmItem> items;
+    };
+    std::vector<Group> groups;
+    for (int j = 0; j < nj; j++) {
+      const chx_job_desc &d = b->jobs[j];
+      const int64_t t0 = (int64_t)r * chunk;
+      if (t0 >= d.time_steps) continue;
+      const int64_t tv = std::min(chunk, d.time_steps - t0);
+      const int nst = (int)(round_up(tv, kRoundSamples) / kMfmaStage);
+      const int KB = nst * (kMfmaStage / 16);
+      const int F = d.num_freq, N = d.num_elements;
+      const int nt = (N + kMfmaTile - 1) / kMfmaTile;
+      const int64_t Npj = round_up(N, kMfmaTile);
+      uint8_t *planes = base + off[j];
+
+      UnpackJob u;
+      u.in = d.d_input;
+      u.planes = planes;
+      u.row_bytes = (int64_t)F * N;
+      u.t0 = t0;
+      u.t_valid = tv;
+      u.unit0 = R.units;
+      u.N = N;
+      u.F = F;
+      u.Np = (int)Npj;
+      u.KB = KB;
+      ujobs.push_back(u);
+      R.units += (int64_t)((Npj + 1023) / 1024) * KB * F;
+
+      // Items of this job: whole chunk per tile, or max_st-stage pieces.
+      const int per = (max_st > 0 && max_st < nst) ? max_st : nst;
+      const int pieces = (nst + per - 1) / per;
+      GemmJob g;
+      g.planes = planes;
+      g.vis = d.d_vis;
+      g.NB = chx_num_blocks(N);
+      g.F = F;
+      g.Np = (int)Npj;
+      g.KB = KB;
+      g.ntiles = nt;
+      g.nb = N / 32;
+      g.U = d.upper_triangle_convention;
+      g.mode = pieces > 1 ? 2 : (r == 0 ? 0 : 1);
+      g.stages_per_split = per;
+      if (pieces > 1 && r == 0) b->zero.push_back({d.d_vis, (int64_t)F * g.NB * 2048});
+      const int gj = (int)gjobs.size();
+      gjobs.push_back(g);
+      const int tiles = nt * (nt + 1) / 2;
+      for (int f = 0; f < F; f++) {
+        Group G;
+        G.work = 0;
+        for (int t = 0; t < tiles; t++)
+          for (int k = 0; k < pieces; k++) {
+            const int s0 = k * per, ns = std::min(per, nst - s0);
+            G.items.push_back({gj, f, t, s0, ns});
+            G.work += ns + 8;  // + a flush/prologue allowance
+          }
+        groups.push_back(std::move(G));
+      }
+    }
+    R.nujobs = (int)ujobs.size() - R.ujob0;
+    // Longest-processing-time first: heaviest group to the lightest queue.
+    std::sort(groups.begin(), groups.end(),
+              [](const Group &a, const Group &c) { return a.work > c.work; });
+    std::vector<std::vector<GemmItem>> q(8);
+    int64_t load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (Group &G : groups) {
+      const int k = (int)(std::min_element(load, load + 8) - load);
+      load[k] += G.work;
+      q[k].insert(q[k].end(), G.items.begin(), G.items.end());
+    }
+    for (int k = 0; k < 8; k++) {
+      std::stable_sort(q[k].begin(), q[k].end(),
+                       [](const GemmItem &a, const GemmItem &c) { return a.nstages > c.nstages; });
+      qbound.push_back((int)items.size());
+      items.insert(items.end(), q[k].begin(), q[k].end());
+    }
+    qbound.push_back((int)items.size());
+    if (items.size() > 0x7FFFFFFFu) return CHX_ERR_SHAPE;
+    b->round.push_back(R);
+  }
+  b->nitems = (int64_t)items.size();
+  return CHX_SUCCESS;
+}
+
+// Invariants the kernels rely on, checked on the host before anything is
+// launched: every plane region inside the workspace, every item inside its
+// job's chunk (whole rings), queues covering the items in order.
+int check_tables(const chx_batch *b, const uint8_t *base, const Tables &t) {
+  const uint8_t *end = base + b->work_bytes;
+  for (const UnpackJob &u : t.ujobs) {
+    const int64_t bytes = 2LL * u.F * u.KB * u.Np * 16;
+    if (!u.in || u.planes < base || u.planes + bytes > end || u.t_valid < 1 ||
+        u.t_valid > 16LL * u.KB || u.Np % kMfmaTile || u.N > u.Np)
+      return CHX_ERR_HIP;
+  }
+  for (const GemmJob &g : t.gjobs) {
+    const int64_t bytes = 2LL * g.F * g.KB * g.Np * 16;
+    if (!g.vis || g.planes < base || g.planes + bytes > end || g.KB % (4 * kMfmaRing) ||
+        g.ntiles * kMfmaTile != g.Np || (int64_t)g.KB * g.Np * 16 > 0x7FFFFFFFLL)
+      return CHX_ERR_HIP;
+  }
+  for (const GemmItem &w : t.items) {
+    if (w.job < 0 || w.job >= (int)t.gjobs.size()) return CHX_ERR_HIP;
+    const GemmJob &g = t.gjobs[w.job];
+    if (w.f < 0 || w.f >= g.F || w.tile < 0 || w.tile >= g.ntiles * (g.ntiles + 1) / 2 ||
+        w.stage0 < 0 || w.nstages < kMfmaRing || w.nstages % kMfmaRing ||
+        w.stage0 + w.nstages > g.KB / 4)
+      return CHX_ERR_HIP;
+  }
+  if ((int)t.qbound.size() != 9 * b->rounds || t.qbound.empty() ||
+      t.qbound.back() != (int)t.items.size() || t.qbound[0] != 0)
+    return CHX_ERR_HIP;
+  for (size_t k = 1; k < t.qbound.size(); k++)
+    if (t.qbound[k] < t.qbound[k - 1] || (k % 9 == 0 && t.qbound[k] != t.qbound[k - 1]))
+      return CHX_ERR_HIP;
+  return CHX_SUCCESS;
+}
+
+// Allocates the workspace, builds and checks the tables, uploads them.
+int plan_batch(chx_batch *b) {
+  std::vector<int64_t> off;
+  int rc = plan_sizes(b, off);
+  if (rc) return rc;
+  CHX_HIP(hipMalloc((void **)&b->d_work, b->work_bytes));
+  Tables t;
+  rc = plan_tables(b, b->d_work, off, t);
+  if (!rc) rc = check_tables(b, b->d_work, t);
+  if (rc) return rc;
+  hipError_t e = upload(&b->d_ujobs, t.ujobs);
+  if (e == hipSuccess) e = upload(&b->d_gjobs, t.gjobs);
+  if (e == hipSuccess) e = upload(&b->d_items, t.items);
+  if (e == hipSuccess) e = upload(&b->d_qbound, t.qbound);
+  if (e == hipSuccess) e = hipMalloc((void **)&b->d_counters, sizeof(int) * 8 * b->rounds);
+  if (e != hipSuccess) return hip_status(e);
+  return CHX_SUCCESS;
+}
+
+// The run's launches, in order, on stream s.
+int enqueue(const chx_batch *b, hipStream_t s) {
+  int rc = launch_fill_u32((uint32_t *)b->d_counters, 0u, 8LL * b->rounds, s);
+  for (const chx_batch::Fill &z : b->zero)
+    if (!rc) rc = launch_fill_u32((uint32_t *)z.p, 0u, z.n, s);
+  for (int r = 0; r < b->rounds && !rc; r++) {
+    const chx_batch::Round &R = b->round[r];
+    rc = launch_unpack_batch(b->d_ujobs + R.ujob0, R.nujobs, R.units, s);
+    if (!rc)
+      rc = launch_gemm_batch(b->d_gjobs, b->d_items, b->d_qbound + 9 * r,
+                             b->d_counters + 8 * r, b->tune.workgroups, s);
+  }
+  return rc;
+}
+
+// Validation shared by chx_batch_create and chx_batch_plan (host only).
+int prepare(const chx_job_desc *jobs, int32_t num_jobs, const chx_batch_tuning *tuning,
+            chx_batch_tuning &t) {
+  if (!jobs || num_jobs < 1) return CHX_ERR_INVALID_ARGUMENT;
+  for (int j = 0; j < num_jobs; j++) {
+    int rc = validate_job(jobs[j]);
+    if (rc) return rc;
+  }
+  t = tuning ? *tuning : chx_batch_tuning{};
+  if (t.max_item_stages < 0 || t.workgroups < 0 || t.workgroups % 8 != 0 ||
+      t.workspace_bytes < 0)
+    return CHX_ERR_INVALID_ARGUMENT;
+  if (t.workspace_bytes == 0) t.workspace_bytes = kDefaultBudget;
+  t.max_item_stages = (int32_t)round_up(t.max_item_stages, kMfmaRing);
+  // The smallest round (256 samples of every job) must fit the budget.
+  int64_t per_sample = 0;
+  for (int j = 0; j < num_jobs; j++)
+    per_sample += 2 * (int64_t)jobs[j].num_freq * round_up(jobs[j].num_elements, kMfmaTile);
+  if (per_sample > t.workspace_bytes / kRoundSamples) return CHX_ERR_BUDGET;
+  return CHX_SUCCESS;
+}
+
+}  // namespace
+}  // namespace chx
+
+using namespace chx;
+
+extern "C" {
+
+int chx_batch_create(int32_t device, const chx_job_desc *jobs, int32_t num_jobs,
+                     const chx_batch_tuning *tuning, chx_batch **out) {
+  if (!out) return CHX_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  chx_batch_tuning t;
+  int rc = prepare(jobs, num_jobs, tuning, t);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return CHX_ERR_NO_DEVICE;
+  if (device < 0 || device >= ndev) return CHX_ERR_NO_DEVICE;
+  CHX_HIP(hipSetDevice(device));
+  if (t.workgroups == 0) {
+    int cus = 0;
+    CHX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    t.workgroups = std::max(8, cus / 8 * 8);
+  }
+  chx_batch *b = new (std::nothrow) chx_batch;
+  if (!b) return CHX_ERR_OUT_OF_MEMORY;
+  b->device = device;
+  b->jobs.assign(jobs, jobs + num_jobs);
+  b->tune = t;
+  rc = plan_batch(b);
+  if (rc) {
+    free_batch(b);
+    return rc;
+  }
+  *out = b;
+  return CHX_SUCCESS;
+}
+
+int chx_batch_plan(const chx_job_desc *jobs, int32_t num_jobs, const chx_batch_tuning *tuning,
+                   chx_batch_info *info) {
+  if (!info) return CHX_ERR_INVALID_ARGUMENT;
+  chx_batch_tuning t;
+  int rc = prepare(jobs, num_jobs, tuning, t);
+  if (rc) return rc;
+  chx_batch b;
+  b.jobs.assign(jobs, jobs + num_jobs);
+  b.tune = t;
+  std::vector<int64_t> off;
+  rc = plan_sizes(&b, off);
+  if (rc) return rc;
+  // Plane pointers relative to a stand-in base (never dereferenced).
+  uint8_t *base = reinterpret_cast<uint8_t *>((uintptr_t)1 << 44);
+  Tables tab;
+  rc = plan_tables(&b, base, off, tab);
+  if (!rc) rc = check_tables(&b, base, tab);
+  if (rc) return rc;
+  return chx_batch_get_info(&b, info);
+}
+
+int chx_batch_run(chx_batch *b, int32_t flags, void *stream) {
+  if (!b || (flags & ~CHX_BATCH_EAGER)) return CHX_ERR_INVALID_ARGUMENT;
+  CHX_HIP(hipSetDevice(b->device));
+  hipStream_t s = (hipStream_t)stream;
+  if (flags & CHX_BATCH_EAGER) return enqueue(b, s);
+  if (!b->exec) {
+    // Captured on the batch's own stream (a legacy default stream cannot
+    // be captured), replayed on the caller's.
+    if (!b->cap) CHX_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+    CHX_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue(b, b->cap);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->cap, &g);
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc ? rc : hip_status(e);
+    }
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      b->exec = nullptr;
+      return hip_status(e);
+    }
+    b->graph = g;
+  }
+  CHX_HIP(hipGraphLaunch(b->exec, s));
+  return CHX_SUCCESS;
+}
+
+int chx_batch_get_info(const chx_batch *b, chx_batch_info *info) {
+  if (!b || !info) return CHX_ERR_INVALID_ARGUMENT;
+  memset(info, 0, sizeof(*info));
+  info->num_jobs = (int32_t)b->jobs.size();
+  info->rounds = b->rounds;
+  info->chunk_samples = b->chunk;
+  info->items = b->nitems;
+  info->workgroups = b->tune.workgroups;
+  info->max_item_stages = b->tune.max_item_stages;
+  info->workspace_bytes = b->work_bytes;
+  info->cmacs = b->cmacs;
+  return CHX_SUCCESS;
+}
+
+int chx_batch_destroy(chx_batch *b) {
+  if (!b) return CHX_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(b->device);
+  free_batch(b);
+  return CHX_SUCCESS;
+}
+
+}  // extern "C"

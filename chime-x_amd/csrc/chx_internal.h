@@ -48,8 +48,43 @@ int launch_blocks_to_triangle(int32_t F, int32_t N, const int32_t *blocks,
 // int8 matrix-core path (chx_mfma.hip).  mode: 0 assign, 1 add.
 // The plane workspace of one time chunk is at most kWorkspaceBytes.
 constexpr int64_t kWorkspaceBytes = 4LL << 30;
+constexpr int kMfmaTile = 128;   // elements per output tile side
+constexpr int kMfmaStage = 64;   // samples per pipeline stage
+constexpr int kMfmaRing = 4;     // stages in flight; stage counts are padded to it
 size_t mfma_workspace_bytes(const chx_plan *p);
 int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis,
                      int mode, hipStream_t s);
+
+// One job as the matrix-core kernels see it (per time chunk).
+struct GemmJob {
+  const uint8_t *planes;  // int8 planes [2][F][KB][Np][16] of the chunk
+  int32_t *vis;           // block-order output
+  int64_t NB;             // blocks per channel
+  int F, Np, KB, ntiles, nb, U, mode, stages_per_split;  // mode 0 =, 1 +=, 2 atomic +=
+};
+// One work item of the persistent batch kernel: tile `tile` (row-major over
+// the upper triangle of tiles) of channel f of jobs[job], over stages
+// [stage0, stage0 + nstages) of the chunk (nstages a multiple of kMfmaRing).
+struct GemmItem {
+  int job, f, tile, stage0, nstages;
+};
+// [T][F][N] bytes (samples t0 .. t0+t_valid) -> planes with KB 16-sample
+// blocks (zero past t_valid and past N).
+int launch_unpack_planes(const uint8_t *in, uint8_t *planes, int N, int F, int Np,
+                         int64_t row_bytes, int64_t t0, int64_t t_valid, int KB,
+                         hipStream_t s);
+// One job's chunk for the batched unpack: units = ceil(Np/1024) * KB * F
+// workgroups starting at global workgroup unit0 (increasing over jobs).
+struct UnpackJob {
+  const uint8_t *in;
+  uint8_t *planes;
+  int64_t row_bytes, t0, t_valid, unit0;
+  int N, F, Np, KB;
+};
+int launch_unpack_batch(const UnpackJob *d_jobs, int njobs, int64_t units, hipStream_t s);
+// Persistent batch GEMM over 8 per-XCD item queues (queue q = items
+// [qbound[q], qbound[q+1])); the 8 counters must be 0.
+int launch_gemm_batch(const GemmJob *d_jobs, const GemmItem *d_items, const int *d_qbound,
+                      int *d_counters, int nwg, hipStream_t s);
 
 }  // namespace chx

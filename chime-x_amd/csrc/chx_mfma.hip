@@ -36,9 +36,9 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) void lds_void;
 
-constexpr int kTile = 128;     // elements per tile side
-constexpr int kStage = 64;     // samples per LDS stage (2 MFMA k-steps)
-constexpr int kRing = 4;       // LDS stages in flight
+constexpr int kTile = kMfmaTile;   // elements per tile side
+constexpr int kStage = kMfmaStage; // samples per LDS stage (2 MFMA k-steps)
+constexpr int kRing = kMfmaRing;    // LDS stages in flight
 constexpr int kChunk = 32768;  // samples accumulated before a flush
 // LDS stage layout: [plane 2][row group 8][16-sample block 4][32 rows][16 B].
 // A 32-row, 32-sample operand is 1 KB contiguous: lanes 0-31 read block
@@ -115,13 +115,13 @@ __device__ __forceinline__ void flush_block(int32_t *blk, const v16i &rea, const
 // bytes per wave and row), four 4x4 byte transposes, plane split; the
 // planes go through LDS so that every global store writes 1 KB of
 // consecutive 16-byte element chunks (full cache lines).
-__global__ __launch_bounds__(256) void unpack_planes_kernel(
-    const uint8_t *__restrict__ in, uint8_t *__restrict__ out_planes, int N, int F, int Np,
-    int64_t row_bytes, int64_t t0, int64_t t_valid, int KB) {
-  __shared__ __attribute__((aligned(16))) v4u stage[2][1024 + 64];  // [plane][element], padded
+__device__ __forceinline__ void unpack_unit(const uint8_t *__restrict__ in,
+                                            uint8_t *__restrict__ out_planes, int N, int F,
+                                            int Np, int64_t row_bytes, int64_t t0,
+                                            int64_t t_valid, int KB, int qb, int kb, int f,
+                                            v4u (*stage)[1024 + 64]) {
   const int tid = threadIdx.x;
-  const int q = blockIdx.x * 256 + tid;  // element quad
-  const int kb = blockIdx.y, f = blockIdx.z;
+  const int q = qb * 256 + tid;  // element quad
   const bool evalid = 4 * q < N;
   uint32_t w[16];
   const uint8_t *src = in + (t0 + 16 * (int64_t)kb) * row_bytes + (int64_t)f * N + 4 * q;
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void unpack_planes_kernel(
   }
   __syncthreads();
   const int64_t plane_stride = (int64_t)F * KB * Np * 16;
-  const int e_base = blockIdx.x * 1024;
+  const int e_base = qb * 1024;
   uint8_t *dst = out_planes + (((int64_t)f * KB + kb) * Np + e_base) * 16;
   const int n_el = Np - e_base < 1024 ? Np - e_base : 1024;
 #pragma unroll
@@ -161,31 +161,54 @@ __global__ __launch_bounds__(256) void unpack_planes_kernel(
   }
 }
 
-// Pass 2.  grid.x = channels x upper-triangle tiles (XCD-remapped),
-// grid.y = time splits of the chunk (each a whole number of stages).
-// mode: 0 assign, 1 add (first flush; later flushes add), 2 atomic add.
-__global__ __launch_bounds__(256, 1) void corr_mfma_kernel(
-    const uint8_t *__restrict__ planes, int32_t *__restrict__ vis, int F, int Np, int KB,
-    int stages_per_split, int ntiles, int64_t NB, int nb, int U, int mode) {
-  __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kStageBytes];
+__global__ __launch_bounds__(256) void unpack_planes_kernel(
+    const uint8_t *__restrict__ in, uint8_t *__restrict__ out_planes, int N, int F, int Np,
+    int64_t row_bytes, int64_t t0, int64_t t_valid, int KB) {
+  __shared__ __attribute__((aligned(16))) v4u stage[2][1024 + 64];  // [plane][element], padded
+  unpack_unit(in, out_planes, N, F, Np, row_bytes, t0, t_valid, KB, blockIdx.x, blockIdx.y,
+              blockIdx.z, stage);
+}
 
+// Batch: one launch unpacks the chunks of many jobs.  Workgroup g belongs
+// to the job j with jobs[j].unit0 <= g < jobs[j+1].unit0 (binary search);
+// its unit index within the job is (channel, block, element group).
+__global__ __launch_bounds__(256) void unpack_batch_kernel(const UnpackJob *__restrict__ jobs,
+                                                           int njobs) {
+  __shared__ __attribute__((aligned(16))) v4u stage[2][1024 + 64];
+  const int64_t g = blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].unit0 <= g) lo = mid; else hi = mid - 1;
+  }
+  const UnpackJob &J = jobs[lo];
+  const int64_t u = g - J.unit0;
+  const int nq = (J.Np + 1023) / 1024;
+  if (u >= (int64_t)nq * J.KB * J.F) return;  // grid padding (uniform per workgroup)
+  const int qb = (int)(u % nq);
+  const int kb = (int)((u / nq) % J.KB);
+  const int f = (int)(u / ((int64_t)nq * J.KB));
+  unpack_unit(J.in, J.planes, J.N, J.F, J.Np, J.row_bytes, J.t0, J.t_valid, J.KB, qb, kb, f,
+              stage);
+}
+
+// Pass 2, one output tile: the (ti, tj) 128x128-element tile of channel f
+// of one job, over stages [st_begin, st_begin + nst) (nst a multiple of
+// kRing).
+// Shared by the per-plan kernel (one tile per workgroup) and the persistent
+// batch kernel (tiles taken from a work list).  J.mode: 0 assign, 1 add
+// (first flush; later flushes of the tile add), 2 atomic add.
+__device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const int ti,
+                                          const int tj, const int st_begin, const int nst,
+                                          uint8_t (*lds)[kStageBytes]) {
+  const int F = J.F, Np = J.Np, KB = J.KB, nb = J.nb, U = J.U, mode = J.mode;
+  const int64_t NB = J.NB;
+  const uint8_t *planes = J.planes;
+  int32_t *vis = J.vis;
   // wave index made provably uniform: descriptors and LDS destinations built
   // from it stay in scalar registers.
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int64_t total = (int64_t)gridDim.x;
-  const int64_t item = xcd_remap(blockIdx.x, total);
-  const int64_t per_f = (int64_t)ntiles * (ntiles + 1) / 2;
-  const int f = (int)(item / per_f);
-  int tb = (int)(item % per_f);
-  int ti = 0;
-  while (tb >= ntiles - ti) {
-    tb -= ntiles - ti;
-    ti++;
-  }
-  const int tj = ti + tb;  // tile column >= tile row
-  const int st_begin = blockIdx.y * stages_per_split;
-  const int nst = stages_per_split;
 
   // Loads: wave w fills the 8 operands (plane w/2, row groups 4(w&1)..+3,
   // k-steps 0/1) of each stage.  One descriptor per (plane, channel); the
@@ -296,8 +319,71 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(
       }
     if (cur_mode == 0) cur_mode = 1;
   }
-  // Drain the trailing (unused) loads before the waves exit.
+  // Drain the trailing (unused) loads: they must not land in a ring slot
+  // after the next tile's first loads.
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+// Tile index tb (row-major over the upper triangle of ntiles x ntiles tiles)
+// -> (ti, tj), tj >= ti.
+__device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti, int &tj) {
+  int r = 0;
+  while (tb >= ntiles - r) {
+    tb -= ntiles - r;
+    r++;
+  }
+  ti = r;
+  tj = r + tb;
+}
+
+// One plan: grid.x = channels x upper-triangle tiles (XCD-remapped), grid.y
+// = time splits.
+__global__ __launch_bounds__(256, 1) void corr_mfma_kernel(GemmJob J) {
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kStageBytes];
+  const int64_t item = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
+  const int64_t per_f = (int64_t)J.ntiles * (J.ntiles + 1) / 2;
+  int ti, tj;
+  tile_coords((int)(item % per_f), J.ntiles, ti, tj);
+  gemm_tile(J, (int)(item / per_f), ti, tj, blockIdx.y * J.stages_per_split,
+            J.stages_per_split, lds);
+}
+
+// Batch: persistent workgroups take work items (job, channel, tile, stage
+// range) from 8 queues, one per XCD (dispatch places workgroup i on XCD
+// i % 8), so that the tiles of one channel run on one XCD and share its L2.
+// A workgroup drains its own queue, then moves on to the next ones; queues
+// never refill, so every workgroup ends after at most nitems items and 8
+// queue moves.  Queue q holds items [qbound[q], qbound[q+1]).
+__global__ __launch_bounds__(256, 1) void corr_mfma_batch_kernel(
+    const GemmJob *__restrict__ jobs, const GemmItem *__restrict__ items,
+    const int *__restrict__ qbound, int *__restrict__ counters) {
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kStageBytes];
+  __shared__ int next;
+  int q = blockIdx.x % 8, moves = 0;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int it = -1;
+      while (moves < 8) {
+        const int k = atomicAdd(&counters[q], 1) + qbound[q];
+        if (k < qbound[q + 1]) {
+          it = k;
+          break;
+        }
+        q = (q + 1) % 8;
+        moves++;
+      }
+      next = it;
+    }
+    __syncthreads();
+    const int it = next;
+    __syncthreads();
+    if (it < 0) break;
+    const GemmItem w = items[it];
+    const GemmJob J = jobs[w.job];
+    int ti, tj;
+    tile_coords(w.tile, J.ntiles, ti, tj);
+    gemm_tile(J, w.f, ti, tj, w.stage0, w.nstages, lds);
+  }
 }
 
 }  // namespace
@@ -336,9 +422,8 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
     // stages, padded to a multiple of the ring (the padding samples are 0)
     const int nst_chunk = (int)((tv + kStage * kRing - 1) / (kStage * kRing)) * kRing;
     const int KB = nst_chunk * (kStage / 16);
-    unpack_planes_kernel<<<dim3((unsigned)((Np + 1023) / 1024), (unsigned)KB, (unsigned)F),
-                           dim3(256), 0, s>>>(in, p->d_work, N, F, Np, row_bytes, t0, tv, KB);
-    CHX_HIP(hipGetLastError());
+    int rc0 = launch_unpack_planes(in, p->d_work, N, F, Np, row_bytes, t0, tv, KB, s);
+    if (rc0) return rc0;
     // Split time only when the tiles alone cannot fill the chip twice over,
     // keeping >= 16 stages per split.
     int split = (int)((512 + items - 1) / items);
@@ -353,10 +438,47 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
       }
       kmode = 2;
     }
-    corr_mfma_kernel<<<dim3((unsigned)items, (unsigned)split), dim3(256), 0, s>>>(
-        p->d_work, vis, F, Np, KB, nst_chunk / split, ntiles, p->NB, p->nb, U, kmode);
+    GemmJob J;
+    J.planes = p->d_work;
+    J.vis = vis;
+    J.NB = p->NB;
+    J.F = F;
+    J.Np = Np;
+    J.KB = KB;
+    J.ntiles = ntiles;
+    J.nb = p->nb;
+    J.U = U;
+    J.mode = kmode;
+    J.stages_per_split = nst_chunk / split;
+    corr_mfma_kernel<<<dim3((unsigned)items, (unsigned)split), dim3(256), 0, s>>>(J);
     CHX_HIP(hipGetLastError());
   }
+  return CHX_SUCCESS;
+}
+
+int launch_unpack_planes(const uint8_t *in, uint8_t *planes, int N, int F, int Np,
+                         int64_t row_bytes, int64_t t0, int64_t t_valid, int KB,
+                         hipStream_t s) {
+  unpack_planes_kernel<<<dim3((unsigned)((Np + 1023) / 1024), (unsigned)KB, (unsigned)F),
+                         dim3(256), 0, s>>>(in, planes, N, F, Np, row_bytes, t0, t_valid, KB);
+  CHX_HIP(hipGetLastError());
+  return CHX_SUCCESS;
+}
+
+int launch_unpack_batch(const UnpackJob *d_jobs, int njobs, int64_t units, hipStream_t s) {
+  if (units <= 0) return CHX_SUCCESS;
+  const int64_t gx = units < 65536 ? units : 65536;
+  const int64_t gy = (units + gx - 1) / gx;
+  unpack_batch_kernel<<<dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s>>>(d_jobs, njobs);
+  CHX_HIP(hipGetLastError());
+  return CHX_SUCCESS;
+}
+
+int launch_gemm_batch(const GemmJob *d_jobs, const GemmItem *d_items, const int *d_qbound,
+                      int *d_counters, int nwg, hipStream_t s) {
+  corr_mfma_batch_kernel<<<dim3((unsigned)nwg), dim3(256), 0, s>>>(d_jobs, d_items, d_qbound,
+                                                                   d_counters);
+  CHX_HIP(hipGetLastError());
   return CHX_SUCCESS;
 }
 

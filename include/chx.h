@@ -134,6 +134,70 @@ int chx_copy_to_host(void *h_dst, const void *d_src, size_t bytes, void *stream)
 /* Waits for all work queued on stream (NULL = default stream). */
 int chx_stream_synchronize(void *stream);
 
+/* ---- Batch: many correlations of uneven shape in one graph launch. ----
+ * No reference counterpart: the reference builds one program per shape and
+ * runs one shape per process (main_wrapper.c:309-312).  Every job uses the
+ * int8 matrix-core method, and a job's visibilities are bit-identical to
+ * what chx_correlate assigns for the same job on its own.  Jobs are fixed
+ * at creation (device pointers included); each run re-reads the inputs and
+ * assigns the outputs.  A batch is not thread-safe; runs of one batch on
+ * different streams must not overlap (they share the workspace). */
+typedef struct chx_job_desc {
+  int32_t num_elements;              /* N: multiple of 32, >= 32 */
+  int32_t num_freq;                  /* F >= 1 */
+  int64_t time_steps;                /* T >= 1 (no time_accum: method 2) */
+  int32_t upper_triangle_convention; /* 1 standard, 0 conjugate */
+  int32_t reserved;                  /* must be 0 */
+  const uint8_t *d_input;            /* [T][F][N] bytes on the batch's device */
+  int32_t *d_vis;                    /* [F][NB][32][32][2] int32 (assigned) */
+} chx_job_desc;
+
+/* Launch shape of a batch; zeros pick the defaults.  Picked per job list
+ * by the autotuner (chime-x_amd/autotune.py). */
+typedef struct chx_batch_tuning {
+  int32_t max_item_stages;  /* time extent of one work item, in 64-sample
+                               stages (rounded up to a multiple of 4);
+                               0 = a whole chunk per item */
+  int32_t workgroups;       /* persistent workgroups (multiple of 8);
+                               0 = one per compute unit */
+  int64_t workspace_bytes;  /* HBM budget of one round's int8 planes;
+                               0 = 16 GiB */
+} chx_batch_tuning;
+
+typedef struct chx_batch_info {
+  int32_t num_jobs;
+  int32_t rounds;          /* time chunks: unpack + matrix-core pass each */
+  int64_t chunk_samples;   /* samples of every job per round */
+  int64_t items;           /* work items over all rounds */
+  int32_t workgroups;
+  int32_t max_item_stages;
+  size_t workspace_bytes;  /* allocated planes workspace */
+  int64_t cmacs;           /* sum over jobs of F * T * N(N+1)/2 */
+} chx_batch_info;
+
+typedef struct chx_batch chx_batch;
+
+/* Validates every job (CHX_ERR_SHAPE / CHX_ERR_INVALID_ARGUMENT, nothing
+ * touches the GPU), plans the rounds and the work items, allocates the
+ * workspace (CHX_ERR_BUDGET when one 256-sample round of all jobs exceeds
+ * workspace_bytes). */
+int chx_batch_create(int32_t device, const chx_job_desc *jobs, int32_t num_jobs,
+                     const chx_batch_tuning *tuning, chx_batch **batch);
+
+/* The same planning on the host only (nothing touches a GPU): validates
+ * like chx_batch_create and reports the rounds, work items and workspace a
+ * batch of these jobs and tuning would use (workgroups 0 = one per compute
+ * unit of the device). */
+int chx_batch_plan(const chx_job_desc *jobs, int32_t num_jobs,
+                   const chx_batch_tuning *tuning, chx_batch_info *info);
+
+/* flags: 0 = replay the batch's hipGraph (captured on first use) on
+ * stream; CHX_BATCH_EAGER = launch the same kernels directly. */
+#define CHX_BATCH_EAGER 1
+int chx_batch_run(chx_batch *batch, int32_t flags, void *stream);
+int chx_batch_get_info(const chx_batch *batch, chx_batch_info *info);
+int chx_batch_destroy(chx_batch *batch);
+
 /* ---- Layout helpers (host only, no GPU needed). ---- */
 
 /* NB = (N/32)(N/32+1)/2 (main_wrapper.c:277). Returns -1 on a bad N. */

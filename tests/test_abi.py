@@ -99,3 +99,104 @@ def test_valid_config_without_gpu_reports_no_device():
     with pytest.raises(chx.ChxError) as e:
         chx.Plan(256, 1, 256)
     assert e.value.status == chx.ERR_NO_DEVICE
+
+
+def _batch_create(jobs, tuning=None, n=None):
+    arr = (chx.JobDesc * max(1, len(jobs)))()
+    for k, j in enumerate(jobs):
+        base = dict(num_elements=256, num_freq=1, time_steps=4096,
+                    upper_triangle_convention=1, reserved=0, d_input=0x1000, d_vis=0x2000)
+        base.update(j)
+        arr[k] = chx.JobDesc(**base)
+    tp = ctypes.byref(chx.BatchTuning(**tuning)) if tuning else None
+    h = ctypes.c_void_p()
+    rc = chx.lib().chx_batch_create(0, arr, len(jobs) if n is None else n, tp, ctypes.byref(h))
+    if rc == 0:
+        chx.lib().chx_batch_destroy(h)
+    return rc
+
+
+@pytest.mark.parametrize("jobs,tuning,status", [
+    ([dict(num_elements=48)], None, chx.ERR_SHAPE),
+    ([dict(), dict(num_freq=0)], None, chx.ERR_SHAPE),
+    ([dict(time_steps=0)], None, chx.ERR_SHAPE),
+    ([dict(upper_triangle_convention=2)], None, chx.ERR_INVALID_ARGUMENT),
+    ([dict(reserved=1)], None, chx.ERR_INVALID_ARGUMENT),
+    ([dict(d_input=None)], None, chx.ERR_INVALID_ARGUMENT),
+    ([dict(d_vis=None)], None, chx.ERR_INVALID_ARGUMENT),
+    ([dict()], dict(workgroups=12), chx.ERR_INVALID_ARGUMENT),
+    ([dict()], dict(max_item_stages=-4), chx.ERR_INVALID_ARGUMENT),
+    # 256 samples of 2 planes x 256 elements are 128 KiB: one byte less fails
+    ([dict()], dict(workspace_bytes=128 * 1024 - 1), chx.ERR_BUDGET),
+    ([dict(num_elements=160), dict(num_elements=32)], dict(workspace_bytes=(512 + 256) * 256 - 1),
+     chx.ERR_BUDGET),
+])
+def test_batch_validation_is_host_side(jobs, tuning, status):
+    assert _batch_create(jobs, tuning) == status
+
+
+def test_batch_null_arguments_and_no_device():
+    L = chx.lib()
+    h = ctypes.c_void_p()
+    assert L.chx_batch_create(0, None, 1, None, ctypes.byref(h)) == chx.ERR_INVALID_ARGUMENT
+    assert _batch_create([dict()], n=0) == chx.ERR_INVALID_ARGUMENT
+    assert L.chx_batch_run(None, 0, None) == chx.ERR_INVALID_ARGUMENT
+    assert L.chx_batch_get_info(None, None) == chx.ERR_INVALID_ARGUMENT
+    assert L.chx_batch_destroy(None) == chx.ERR_INVALID_ARGUMENT
+    if chx.device_count() == 0:
+        assert _batch_create([dict(), dict(num_elements=2048, num_freq=8)]) == chx.ERR_NO_DEVICE
+        assert _batch_create([dict()], dict(workspace_bytes=128 * 1024)) == chx.ERR_NO_DEVICE
+
+
+def _fake(jobs):
+    return [dict(j, d_input=0x1000, d_vis=0x2000) for j in jobs]
+
+
+def _expected_items(jobs, chunk, max_st):
+    """Work items of the plan: per round, per active job, F x tiles x pieces."""
+    n, rounds = 0, -(-max(j["time_steps"] for j in jobs) // chunk)
+    for r in range(rounds):
+        for j in jobs:
+            t0 = r * chunk
+            if t0 >= j["time_steps"]:
+                continue
+            nst = -(-min(chunk, j["time_steps"] - t0) // 256) * 4
+            per = max_st if 0 < max_st < nst else nst
+            nt = -(-j["num_elements"] // 128)
+            n += j["num_freq"] * nt * (nt + 1) // 2 * -(-nst // per)
+    return rounds, n
+
+
+@pytest.mark.parametrize("tuning", [None, dict(max_item_stages=64), dict(max_item_stages=6),
+                                    dict(workspace_bytes=1 << 30),
+                                    dict(workspace_bytes=3 << 30, max_item_stages=512)])
+def test_batch_plan_host_only(tuning):
+    import workloads as W
+    jobs = W.uneven_batch_jobs()
+    info = chx.batch_plan(_fake(jobs), tuning)
+    t = tuning or {}
+    per_sample = sum(2 * j["num_freq"] * -(-j["num_elements"] // 128) * 128 for j in jobs)
+    budget = t.get("workspace_bytes", 16 << 30)
+    maxT = max(j["time_steps"] for j in jobs)
+    chunk = min(budget // per_sample // 256 * 256, -(-maxT // 256) * 256)
+    max_st = -(-t.get("max_item_stages", 0) // 4) * 4
+    rounds, items = _expected_items(jobs, chunk, max_st)
+    assert info["chunk_samples"] == chunk
+    assert info["rounds"] == rounds and info["items"] == items
+    assert info["max_item_stages"] == max_st
+    assert info["workspace_bytes"] <= budget
+    assert info["cmacs"] == sum(W.cmacs(j["num_elements"], j["num_freq"], j["time_steps"])
+                                for j in jobs)
+    assert info["num_jobs"] == 64
+
+
+def test_batch_plan_validates_like_create():
+    with pytest.raises(chx.ChxError) as e:
+        chx.batch_plan(_fake([dict(num_elements=40, num_freq=1, time_steps=64)]))
+    assert e.value.status == chx.ERR_SHAPE
+    with pytest.raises(chx.ChxError) as e:
+        chx.batch_plan(_fake([dict(num_elements=256, num_freq=1, time_steps=64)]),
+                       dict(workspace_bytes=1000))
+    assert e.value.status == chx.ERR_BUDGET
+    one = chx.batch_plan(_fake([dict(num_elements=32, num_freq=1, time_steps=1)]))
+    assert one["rounds"] == 1 and one["items"] == 1 and one["chunk_samples"] == 256
