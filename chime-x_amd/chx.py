@@ -94,6 +94,35 @@ class BatchInfo(ctypes.Structure):
 
 BATCH_EAGER = 1
 
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32),
+        ("frame_slots", ctypes.c_int32),
+        ("slot_bytes", ctypes.c_int64),
+        ("workspace_bytes", ctypes.c_int64),
+        ("hbm_budget_bytes", ctypes.c_int64),
+        ("max_batch_frames", ctypes.c_int32),
+        ("max_item_stages", ctypes.c_int32),
+    ]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [
+        ("frames_submitted", ctypes.c_int64),
+        ("frames_done", ctypes.c_int64),
+        ("batches", ctypes.c_int64),
+        ("streams_open", ctypes.c_int64),
+        ("bytes_in_use", ctypes.c_int64),
+        ("wall_seconds", ctypes.c_double),
+        ("cmacs", ctypes.c_double),
+        ("mean_batch_frames", ctypes.c_double),
+        ("latency_p50_ms", ctypes.c_double),
+        ("latency_p99_ms", ctypes.c_double),
+        ("latency_max_ms", ctypes.c_double),
+        ("latency_mean_ms", ctypes.c_double),
+    ]
+
 _lib = None
 
 
@@ -132,6 +161,19 @@ def lib() -> ctypes.CDLL:
     L.chx_batch_run.argtypes = [vp, i32, vp]
     L.chx_batch_get_info.argtypes = [vp, ctypes.POINTER(BatchInfo)]
     L.chx_batch_destroy.argtypes = [vp]
+    pi64 = ctypes.POINTER(ctypes.c_int64)
+    L.chx_serve_create.argtypes = [ctypes.POINTER(ServeConfig), ctypes.POINTER(vp)]
+    L.chx_serve_destroy.argtypes = [vp]
+    L.chx_serve_open.argtypes = [vp, i32, i32, i32, i32, i64, pi64]
+    L.chx_serve_submit.argtypes = [vp, i64, vp, pi64]
+    L.chx_serve_submit_device.argtypes = [vp, i64, vp, pi64]
+    L.chx_serve_wait_frame.argtypes = [vp, i64]
+    L.chx_serve_read.argtypes = [vp, i64, vp]
+    L.chx_serve_read_device.argtypes = [vp, i64, vp]
+    L.chx_serve_close.argtypes = [vp, i64]
+    L.chx_serve_get_stats.argtypes = [vp, ctypes.POINTER(ServeStats)]
+    L.chx_serve_reset_stats.argtypes = [vp]
+    L.chx_serve_latencies.argtypes = [vp, vp, vp, i64, pi64]
     _lib = L
     return L
 
@@ -308,6 +350,108 @@ class Batch:
     def close(self) -> None:
         if getattr(self, "_h", None):
             lib().chx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Serve:
+    """Streaming engine with continuous batching (chx_serve).
+
+    Streams are integrations with device-resident accumulators; frames of
+    any open stream are submitted from host memory (copied) or device memory
+    (read in place) and integrated in shared launches.  submit/submit_device
+    return None instead of raising when no frame slot is free (ERR_BUSY),
+    so callers can retry; open returns None when the HBM budget is full.
+    """
+
+    def __init__(self, frame_slots: int, slot_bytes: int, workspace_bytes: int = 0,
+                 hbm_budget_bytes: int = 0, max_batch_frames: int = 0,
+                 max_item_stages: int = 0, device: int = 0):
+        self.cfg = ServeConfig(device, frame_slots, slot_bytes, workspace_bytes,
+                               hbm_budget_bytes, max_batch_frames, max_item_stages)
+        h = ctypes.c_void_p()
+        _check(lib().chx_serve_create(ctypes.byref(self.cfg), ctypes.byref(h)),
+               "chx_serve_create")
+        self._h = h
+        self.shapes = {}
+
+    def open(self, num_elements: int, num_freq: int, frame_samples: int,
+             upper_triangle_convention: int = 1, integration_frames: int = 0):
+        sid = ctypes.c_int64()
+        rc = lib().chx_serve_open(self._h, num_elements, num_freq, frame_samples,
+                                  upper_triangle_convention, integration_frames,
+                                  ctypes.byref(sid))
+        if rc == ERR_BUDGET:
+            return None
+        _check(rc, "chx_serve_open")
+        self.shapes[sid.value] = (num_freq, num_blocks(num_elements), 32, 32, 2)
+        return sid.value
+
+    def _submit(self, fn, sid: int, ptr: int):
+        fid = ctypes.c_int64()
+        rc = fn(self._h, sid, ptr, ctypes.byref(fid))
+        if rc == ERR_BUSY:
+            return None
+        _check(rc, fn.__name__)
+        return fid.value
+
+    def submit(self, sid: int, h_frame) -> int | None:
+        """h_frame: a host address (int) or a C-contiguous uint8 ndarray that
+        the caller keeps unchanged until the frame is done."""
+        ptr = _p(h_frame) if isinstance(h_frame, np.ndarray) else h_frame
+        return self._submit(lib().chx_serve_submit, sid, ptr)
+
+    def submit_device(self, sid: int, d_frame: int) -> int | None:
+        return self._submit(lib().chx_serve_submit_device, sid, d_frame)
+
+    def wait_frame(self, fid: int) -> None:
+        _check(lib().chx_serve_wait_frame(self._h, fid), "chx_serve_wait_frame")
+
+    def read(self, sid: int) -> np.ndarray:
+        out = np.empty(self.shapes[sid], np.int32)
+        _check(lib().chx_serve_read(self._h, sid, _p(out)), "chx_serve_read")
+        return out
+
+    def read_device(self, sid: int, d_vis: int) -> None:
+        _check(lib().chx_serve_read_device(self._h, sid, d_vis), "chx_serve_read_device")
+
+    def close_stream(self, sid: int) -> None:
+        _check(lib().chx_serve_close(self._h, sid), "chx_serve_close")
+        self.shapes.pop(sid, None)
+
+    def stats(self) -> dict:
+        st = ServeStats()
+        _check(lib().chx_serve_get_stats(self._h, ctypes.byref(st)), "chx_serve_get_stats")
+        return {name: getattr(st, name) for name, _ in ServeStats._fields_}
+
+    def reset_stats(self) -> None:
+        _check(lib().chx_serve_reset_stats(self._h), "chx_serve_reset_stats")
+
+    def latencies(self):
+        """(frame ids, submit-to-done ms) of the completed frames."""
+        n = ctypes.c_int64()
+        _check(lib().chx_serve_latencies(self._h, None, None, 0, ctypes.byref(n)),
+               "chx_serve_latencies")
+        ids = np.empty(n.value, np.int64)
+        ms = np.empty(n.value, np.float64)
+        _check(lib().chx_serve_latencies(self._h, _p(ids), _p(ms), n.value, ctypes.byref(n)),
+               "chx_serve_latencies")
+        return ids[:n.value], ms[:n.value]
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().chx_serve_destroy(self._h)
             self._h = None
 
     def __del__(self):

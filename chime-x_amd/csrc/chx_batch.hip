@@ -52,6 +52,81 @@ struct chx_batch {
 };
 
 namespace chx {
+
+int add_job_items(int gj, int F, int ntiles, int nst, int max_st, std::vector<ItemGroup> &groups) {
+  const int per = (max_st > 0 && max_st < nst) ? max_st : nst;
+  const int pieces = (nst + per - 1) / per;
+  const int tiles = ntiles * (ntiles + 1) / 2;
+  for (int f = 0; f < F; f++) {
+    ItemGroup G;
+    G.work = 0;
+    G.items.reserve((size_t)tiles * pieces);
+    for (int t = 0; t < tiles; t++)
+      for (int k = 0; k < pieces; k++) {
+        const int s0 = k * per, ns = per < nst - s0 ? per : nst - s0;
+        G.items.push_back({gj, f, t, s0, ns});
+        G.work += ns + 8;  // + a flush/prologue allowance
+      }
+    groups.push_back(std::move(G));
+  }
+  return pieces;
+}
+
+void spread_over_queues(std::vector<ItemGroup> &groups, std::vector<GemmItem> &items,
+                        std::vector<int> &qbound) {
+  // Longest-processing-time first: heaviest group to the lightest queue.
+  std::stable_sort(groups.begin(), groups.end(),
+                   [](const ItemGroup &a, const ItemGroup &c) { return a.work > c.work; });
+  std::vector<std::vector<GemmItem>> q(8);
+  int64_t load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (ItemGroup &G : groups) {
+    const int k = (int)(std::min_element(load, load + 8) - load);
+    load[k] += G.work;
+    q[k].insert(q[k].end(), G.items.begin(), G.items.end());
+  }
+  for (int k = 0; k < 8; k++) {
+    std::stable_sort(q[k].begin(), q[k].end(),
+                     [](const GemmItem &a, const GemmItem &c) { return a.nstages > c.nstages; });
+    qbound.push_back((int)items.size());
+    items.insert(items.end(), q[k].begin(), q[k].end());
+  }
+  qbound.push_back((int)items.size());
+}
+
+int check_launch_tables(const std::vector<UnpackJob> &ujobs, const std::vector<GemmJob> &gjobs,
+                        const std::vector<GemmItem> &items, const std::vector<int> &qbound,
+                        int rounds, const uint8_t *base, size_t work_bytes) {
+  const uint8_t *end = base + work_bytes;
+  for (const UnpackJob &u : ujobs) {
+    const int64_t bytes = 2LL * u.F * u.KB_dst * u.Np * 16;
+    if (!u.in || u.planes < base || u.planes + bytes > end || u.t_valid < 1 ||
+        u.t_valid > 16LL * u.KB || u.Np % kMfmaTile || u.N > u.Np || u.kb_dst0 < 0 ||
+        u.kb_dst0 + u.KB > u.KB_dst)
+      return CHX_ERR_HIP;
+  }
+  for (const GemmJob &g : gjobs) {
+    const int64_t bytes = 2LL * g.F * g.KB * g.Np * 16;
+    if (!g.vis || g.planes < base || g.planes + bytes > end || g.KB % (4 * kMfmaRing) ||
+        g.ntiles * kMfmaTile != g.Np || (int64_t)g.KB * g.Np * 16 > 0x7FFFFFFFLL)
+      return CHX_ERR_HIP;
+  }
+  for (const GemmItem &w : items) {
+    if (w.job < 0 || w.job >= (int)gjobs.size()) return CHX_ERR_HIP;
+    const GemmJob &g = gjobs[w.job];
+    if (w.f < 0 || w.f >= g.F || w.tile < 0 || w.tile >= g.ntiles * (g.ntiles + 1) / 2 ||
+        w.stage0 < 0 || w.nstages < kMfmaRing || w.nstages % kMfmaRing ||
+        w.stage0 + w.nstages > g.KB / 4)
+      return CHX_ERR_HIP;
+  }
+  if ((int)qbound.size() != 9 * rounds || qbound.empty() ||
+      qbound.back() != (int)items.size() || qbound[0] != 0)
+    return CHX_ERR_HIP;
+  for (size_t k = 1; k < qbound.size(); k++)
+    if (qbound[k] < qbound[k - 1] || (k % 9 == 0 && qbound[k] != qbound[k - 1]))
+      return CHX_ERR_HIP;
+  return CHX_SUCCESS;
+}
+
 namespace {
 
 constexpr int64_t kRoundSamples = (int64_t)kMfmaStage * kMfmaRing;  // 256
@@ -151,11 +226,7 @@ int plan_tables(chx_batch *b, uint8_t *base, const std::vector<int64_t> &off, Ta
   for (int r = 0; r < b->rounds; r++) {
     chx_batch::Round R;
     R.ujob0 = (int)ujobs.size();
-    struct Group {
-      int64_t work;
-      std::vector<GemmItem> items;
-    };
-    std::vector<Group> groups;
+    std::vector<ItemGroup> groups;
     for (int j = 0; j < nj; j++) {
       const chx_job_desc &d = b->jobs[j];
       const int64_t t0 = (int64_t)r * chunk;
@@ -179,12 +250,14 @@ int plan_tables(chx_batch *b, uint8_t *base, const std::vector<int64_t> &off, Ta
       u.F = F;
       u.Np = (int)Npj;
       u.KB = KB;
+      u.KB_dst = KB;
+      u.kb_dst0 = 0;
       ujobs.push_back(u);
       R.units += (int64_t)((Npj + 1023) / 1024) * KB * F;
 
       // Items of this job: whole chunk per tile, or max_st-stage pieces.
       const int per = (max_st > 0 && max_st < nst) ? max_st : nst;
-      const int pieces = (nst + per - 1) / per;
+      const int pieces = add_job_items((int)gjobs.size(), F, nt, nst, max_st, groups);
       GemmJob g;
       g.planes = planes;
       g.vis = d.d_vis;
@@ -198,39 +271,10 @@ int plan_tables(chx_batch *b, uint8_t *base, const std::vector<int64_t> &off, Ta
       g.mode = pieces > 1 ? 2 : (r == 0 ? 0 : 1);
       g.stages_per_split = per;
       if (pieces > 1 && r == 0) b->zero.push_back({d.d_vis, (int64_t)F * g.NB * 2048});
-      const int gj = (int)gjobs.size();
       gjobs.push_back(g);
-      const int tiles = nt * (nt + 1) / 2;
-      for (int f = 0; f < F; f++) {
-        Group G;
-        G.work = 0;
-        for (int t = 0; t < tiles; t++)
-          for (int k = 0; k < pieces; k++) {
-            const int s0 = k * per, ns = std::min(per, nst - s0);
-            G.items.push_back({gj, f, t, s0, ns});
-            G.work += ns + 8;  // + a flush/prologue allowance
-          }
-        groups.push_back(std::move(G));
-      }
     }
     R.nujobs = (int)ujobs.size() - R.ujob0;
-    // Longest-processing-time first: heaviest group to the lightest queue.
-    std::sort(groups.begin(), groups.end(),
-              [](const Group &a, const Group &c) { return a.work > c.work; });
-    std::vector<std::vector<GemmItem>> q(8);
-    int64_t load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (Group &G : groups) {
-      const int k = (int)(std::min_element(load, load + 8) - load);
-      load[k] += G.work;
-      q[k].insert(q[k].end(), G.items.begin(), G.items.end());
-    }
-    for (int k = 0; k < 8; k++) {
-      std::stable_sort(q[k].begin(), q[k].end(),
-                       [](const GemmItem &a, const GemmItem &c) { return a.nstages > c.nstages; });
-      qbound.push_back((int)items.size());
-      items.insert(items.end(), q[k].begin(), q[k].end());
-    }
-    qbound.push_back((int)items.size());
+    spread_over_queues(groups, items, qbound);
     if (items.size() > 0x7FFFFFFFu) return CHX_ERR_SHAPE;
     b->round.push_back(R);
   }
@@ -238,38 +282,9 @@ int plan_tables(chx_batch *b, uint8_t *base, const std::vector<int64_t> &off, Ta
   return CHX_SUCCESS;
 }
 
-// Invariants the kernels rely on, checked on the host before anything is
-// launched: every plane region inside the workspace, every item inside its
-// job's chunk (whole rings), queues covering the items in order.
 int check_tables(const chx_batch *b, const uint8_t *base, const Tables &t) {
-  const uint8_t *end = base + b->work_bytes;
-  for (const UnpackJob &u : t.ujobs) {
-    const int64_t bytes = 2LL * u.F * u.KB * u.Np * 16;
-    if (!u.in || u.planes < base || u.planes + bytes > end || u.t_valid < 1 ||
-        u.t_valid > 16LL * u.KB || u.Np % kMfmaTile || u.N > u.Np)
-      return CHX_ERR_HIP;
-  }
-  for (const GemmJob &g : t.gjobs) {
-    const int64_t bytes = 2LL * g.F * g.KB * g.Np * 16;
-    if (!g.vis || g.planes < base || g.planes + bytes > end || g.KB % (4 * kMfmaRing) ||
-        g.ntiles * kMfmaTile != g.Np || (int64_t)g.KB * g.Np * 16 > 0x7FFFFFFFLL)
-      return CHX_ERR_HIP;
-  }
-  for (const GemmItem &w : t.items) {
-    if (w.job < 0 || w.job >= (int)t.gjobs.size()) return CHX_ERR_HIP;
-    const GemmJob &g = t.gjobs[w.job];
-    if (w.f < 0 || w.f >= g.F || w.tile < 0 || w.tile >= g.ntiles * (g.ntiles + 1) / 2 ||
-        w.stage0 < 0 || w.nstages < kMfmaRing || w.nstages % kMfmaRing ||
-        w.stage0 + w.nstages > g.KB / 4)
-      return CHX_ERR_HIP;
-  }
-  if ((int)t.qbound.size() != 9 * b->rounds || t.qbound.empty() ||
-      t.qbound.back() != (int)t.items.size() || t.qbound[0] != 0)
-    return CHX_ERR_HIP;
-  for (size_t k = 1; k < t.qbound.size(); k++)
-    if (t.qbound[k] < t.qbound[k - 1] || (k % 9 == 0 && t.qbound[k] != t.qbound[k - 1]))
-      return CHX_ERR_HIP;
-  return CHX_SUCCESS;
+  return check_launch_tables(t.ujobs, t.gjobs, t.items, t.qbound, b->rounds, base,
+                             b->work_bytes);
 }
 
 // Allocates the workspace, builds and checks the tables, uploads them.

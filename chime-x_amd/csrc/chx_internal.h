@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "chx.h"
 
 struct chx_plan {
@@ -75,13 +77,37 @@ int launch_unpack_planes(const uint8_t *in, uint8_t *planes, int N, int F, int N
                          hipStream_t s);
 // One job's chunk for the batched unpack: units = ceil(Np/1024) * KB * F
 // workgroups starting at global workgroup unit0 (increasing over jobs).
+// Its KB 16-sample blocks land at blocks [kb_dst0, kb_dst0 + KB) of planes
+// laid out as [2][F][KB_dst][Np][16].
 struct UnpackJob {
   const uint8_t *in;
   uint8_t *planes;
   int64_t row_bytes, t0, t_valid, unit0;
-  int N, F, Np, KB;
+  int N, F, Np, KB, KB_dst, kb_dst0;
 };
 int launch_unpack_batch(const UnpackJob *d_jobs, int njobs, int64_t units, hipStream_t s);
+// Work items of one (job, channel) group; groups are spread over the 8
+// per-XCD queues by spread_over_queues (chx_batch.hip).
+struct ItemGroup {
+  int64_t work;
+  std::vector<GemmItem> items;
+};
+// Appends the groups of GEMM job gj (F channels, ntiles tiles per side, nst
+// stages cut into pieces of max_st stages, 0 = whole): one group per channel.
+// Returns the number of pieces per tile.
+int add_job_items(int gj, int F, int ntiles, int nst, int max_st, std::vector<ItemGroup> &groups);
+// Longest-processing-time-first assignment of the groups to 8 queues (long
+// items first inside a queue); appends the items to `items` and the 9 queue
+// bounds to `qbound`.
+void spread_over_queues(std::vector<ItemGroup> &groups, std::vector<GemmItem> &items,
+                        std::vector<int> &qbound);
+// Invariants the kernels rely on, checked on the host before a launch:
+// every plane region inside the workspace [base, base + work_bytes), every
+// item inside its job's chunk (whole rings), `rounds` sets of 9 queue bounds
+// covering the items in order.  CHX_ERR_HIP if any fails.
+int check_launch_tables(const std::vector<UnpackJob> &ujobs, const std::vector<GemmJob> &gjobs,
+                        const std::vector<GemmItem> &items, const std::vector<int> &qbound,
+                        int rounds, const uint8_t *base, size_t work_bytes);
 // Persistent batch GEMM over 8 per-XCD item queues (queue q = items
 // [qbound[q], qbound[q+1])); the 8 counters must be 0.
 int launch_gemm_batch(const GemmJob *d_jobs, const GemmItem *d_items, const int *d_qbound,

@@ -115,19 +115,22 @@ __device__ __forceinline__ void flush_block(int32_t *blk, const v16i &rea, const
 // bytes per wave and row), four 4x4 byte transposes, plane split; the
 // planes go through LDS so that every global store writes 1 KB of
 // consecutive 16-byte element chunks (full cache lines).
+// KB is the destination's block count per channel (its layout), kb the
+// destination block written (kb - kb_src0 the source's 16-sample block).
 __device__ __forceinline__ void unpack_unit(const uint8_t *__restrict__ in,
                                             uint8_t *__restrict__ out_planes, int N, int F,
                                             int Np, int64_t row_bytes, int64_t t0,
-                                            int64_t t_valid, int KB, int qb, int kb, int f,
-                                            v4u (*stage)[1024 + 64]) {
+                                            int64_t t_valid, int KB, int qb, int kb,
+                                            int kb_src0, int f, v4u (*stage)[1024 + 64]) {
   const int tid = threadIdx.x;
   const int q = qb * 256 + tid;  // element quad
   const bool evalid = 4 * q < N;
   uint32_t w[16];
-  const uint8_t *src = in + (t0 + 16 * (int64_t)kb) * row_bytes + (int64_t)f * N + 4 * q;
+  const int64_t ks = kb - kb_src0;  // source block
+  const uint8_t *src = in + (t0 + 16 * ks) * row_bytes + (int64_t)f * N + 4 * q;
 #pragma unroll
   for (int s = 0; s < 16; s++) {
-    const bool ok = evalid && (16 * (int64_t)kb + s < t_valid);
+    const bool ok = evalid && (16 * ks + s < t_valid);
     w[s] = ok ? *(const uint32_t *)(src + s * row_bytes) : 0x88888888u;
   }
   uint32_t c[4][4];  // [element][4-sample group]
@@ -165,13 +168,15 @@ __global__ __launch_bounds__(256) void unpack_planes_kernel(
     const uint8_t *__restrict__ in, uint8_t *__restrict__ out_planes, int N, int F, int Np,
     int64_t row_bytes, int64_t t0, int64_t t_valid, int KB) {
   __shared__ __attribute__((aligned(16))) v4u stage[2][1024 + 64];  // [plane][element], padded
-  unpack_unit(in, out_planes, N, F, Np, row_bytes, t0, t_valid, KB, blockIdx.x, blockIdx.y,
+  unpack_unit(in, out_planes, N, F, Np, row_bytes, t0, t_valid, KB, blockIdx.x, blockIdx.y, 0,
               blockIdx.z, stage);
 }
 
 // Batch: one launch unpacks the chunks of many jobs.  Workgroup g belongs
 // to the job j with jobs[j].unit0 <= g < jobs[j+1].unit0 (binary search);
-// its unit index within the job is (channel, block, element group).
+// its unit index within the job is (channel, block, element group).  A job
+// writes blocks [kb_dst0, kb_dst0 + KB) of a destination laid out with
+// KB_dst blocks per channel (several frames of one stream side by side).
 __global__ __launch_bounds__(256) void unpack_batch_kernel(const UnpackJob *__restrict__ jobs,
                                                            int njobs) {
   __shared__ __attribute__((aligned(16))) v4u stage[2][1024 + 64];
@@ -188,8 +193,8 @@ __global__ __launch_bounds__(256) void unpack_batch_kernel(const UnpackJob *__re
   const int qb = (int)(u % nq);
   const int kb = (int)((u / nq) % J.KB);
   const int f = (int)(u / ((int64_t)nq * J.KB));
-  unpack_unit(J.in, J.planes, J.N, J.F, J.Np, J.row_bytes, J.t0, J.t_valid, J.KB, qb, kb, f,
-              stage);
+  unpack_unit(J.in, J.planes, J.N, J.F, J.Np, J.row_bytes, J.t0, J.t_valid, J.KB_dst, qb,
+              J.kb_dst0 + kb, J.kb_dst0, f, stage);
 }
 
 // Pass 2, one output tile: the (ti, tj) 128x128-element tile of channel f

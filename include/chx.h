@@ -198,6 +198,89 @@ int chx_batch_run(chx_batch *batch, int32_t flags, void *stream);
 int chx_batch_get_info(const chx_batch *batch, chx_batch_info *info);
 int chx_batch_destroy(chx_batch *batch);
 
+/* ---- Streaming serve: continuous batching of arriving frames. ----
+ * Replaces the reference's fixed pipeline loop (main_wrapper.c:613-795,
+ * one shape, H2D -> zero -> offset -> preseed -> corr per iteration) with
+ * an engine that keeps accumulators device-resident per stream (one
+ * integration of one array), admits streams by HBM budget and frames by
+ * free frame slots, and integrates every frame that is on the device when
+ * the GPU comes free in one launch (frames of many streams together; a
+ * stream's frames as one longer chunk).  Frames copy in on their own stream
+ * while earlier frames compute.  Results are bit-identical to correlating
+ * each stream's frames alone (int8 matrix-core method, 2^32-wrapping int32
+ * sums).  All entry points are thread-safe. */
+typedef struct chx_serve_config {
+  int32_t device;
+  int32_t frame_slots;      /* frames in flight at most (copying, queued or
+                               computing); a submit needs a free slot */
+  int64_t slot_bytes;       /* device buffer per slot: the largest frame */
+  int64_t workspace_bytes;  /* int8 planes of one launch (2 bytes per input
+                               byte when N % 128 == 0 and frame_samples %
+                               256 == 0); 0 = 2 * max_batch_frames *
+                               slot_bytes */
+  int64_t hbm_budget_bytes; /* slots + workspace + accumulators; 0 = no cap */
+  int32_t max_batch_frames; /* frames per launch at most; 0 = frame_slots */
+  int32_t max_item_stages;  /* as in chx_batch_tuning; 0 = whole */
+} chx_serve_config;
+
+typedef struct chx_serve_stats {
+  int64_t frames_submitted;
+  int64_t frames_done;
+  int64_t batches;           /* launches */
+  int64_t streams_open;
+  int64_t bytes_in_use;      /* slots + workspace + open accumulators */
+  double wall_seconds;       /* first submit -> last completion */
+  double cmacs;              /* algorithmic complex MACs of the frames done */
+  double mean_batch_frames;
+  double latency_p50_ms;     /* frame-to-visibility: submit -> its products */
+  double latency_p99_ms;     /*   are in the stream's accumulator */
+  double latency_max_ms;
+  double latency_mean_ms;
+} chx_serve_stats;
+
+typedef struct chx_serve chx_serve;
+
+int chx_serve_create(const chx_serve_config *cfg, chx_serve **serve);
+/* Finishes every submitted frame, then frees everything. */
+int chx_serve_destroy(chx_serve *serve);
+
+/* Admits a stream: frames of frame_samples x num_freq x num_elements bytes,
+ * integrated into an accumulator that starts at zero.  CHX_ERR_BUDGET when
+ * its accumulator does not fit the budget now (retry after a close), or a
+ * frame does not fit a slot or the workspace.  integration_frames > 0 caps
+ * the frames the stream accepts (0 = no cap). */
+int chx_serve_open(chx_serve *serve, int32_t num_elements, int32_t num_freq,
+                   int32_t frame_samples, int32_t upper_triangle_convention,
+                   int64_t integration_frames, int64_t *stream_id);
+/* Next frame of a stream from host memory: queued for an asynchronous copy
+ * into a free slot.  h_frame must stay unchanged until the frame is done
+ * (chx_serve_wait_frame); pinned memory (chx_host_alloc) lets the copy
+ * overlap computation.  CHX_ERR_BUSY when no slot is free (retry),
+ * CHX_ERR_BUDGET when the stream's integration_frames are all submitted. */
+int chx_serve_submit(chx_serve *serve, int64_t stream_id, const uint8_t *h_frame,
+                     int64_t *frame_id);
+/* The same for a frame already in device memory: read in place (no copy).
+ * Its data must be complete when submitted (the work writing it finished)
+ * and stay valid until the frame is done.  Takes a slot as well. */
+int chx_serve_submit_device(chx_serve *serve, int64_t stream_id,
+                            const uint8_t *d_frame, int64_t *frame_id);
+/* Blocks until the frame's products are in its stream's accumulator. */
+int chx_serve_wait_frame(chx_serve *serve, int64_t frame_id);
+/* Blocks until every frame submitted to the stream is done, then copies its
+ * accumulator ([F][NB][32][32][2] int32, block order) to host or device. */
+int chx_serve_read(chx_serve *serve, int64_t stream_id, int32_t *h_vis);
+int chx_serve_read_device(chx_serve *serve, int64_t stream_id, int32_t *d_vis);
+/* Waits for the stream's frames, releases its accumulator. */
+int chx_serve_close(chx_serve *serve, int64_t stream_id);
+int chx_serve_get_stats(chx_serve *serve, chx_serve_stats *stats);
+/* Clears the latency records and counters (not the streams). */
+int chx_serve_reset_stats(chx_serve *serve);
+/* Latency records of completed frames, oldest first: frame ids and
+ * submit-to-done milliseconds.  Writes at most max records (either array
+ * may be NULL) and the number of records held into *count. */
+int chx_serve_latencies(chx_serve *serve, int64_t *frame_ids, double *ms, int64_t max,
+                        int64_t *count);
+
 /* ---- Layout helpers (host only, no GPU needed). ---- */
 
 /* NB = (N/32)(N/32+1)/2 (main_wrapper.c:277). Returns -1 on a bad N. */

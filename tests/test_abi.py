@@ -200,3 +200,39 @@ def test_batch_plan_validates_like_create():
     assert e.value.status == chx.ERR_BUDGET
     one = chx.batch_plan(_fake([dict(num_elements=32, num_freq=1, time_steps=1)]))
     assert one["rounds"] == 1 and one["items"] == 1 and one["chunk_samples"] == 256
+
+
+def test_serve_validation_is_host_side():
+    L = chx.lib()
+    h = ctypes.c_void_p()
+    assert L.chx_serve_create(None, ctypes.byref(h)) == chx.ERR_INVALID_ARGUMENT
+
+    def create(**kw):
+        base = dict(device=0, frame_slots=4, slot_bytes=1 << 20, workspace_bytes=0,
+                    hbm_budget_bytes=0, max_batch_frames=0, max_item_stages=0)
+        base.update(kw)
+        cfg = chx.ServeConfig(**base)
+        rc = L.chx_serve_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc == 0:
+            L.chx_serve_destroy(h)
+        return rc
+
+    assert create(frame_slots=0) == chx.ERR_INVALID_ARGUMENT
+    assert create(slot_bytes=-1) == chx.ERR_INVALID_ARGUMENT
+    assert create(max_item_stages=-4) == chx.ERR_INVALID_ARGUMENT
+    # 4 slots of 1 MiB + 2 * 4 * 1 MiB of workspace = 12 MiB
+    assert create(hbm_budget_bytes=12 * (1 << 20) - 1) == chx.ERR_BUDGET
+    if chx.device_count() == 0:
+        assert create(hbm_budget_bytes=12 * (1 << 20)) == chx.ERR_NO_DEVICE
+    sid = ctypes.c_int64()
+    for fn, args in ((L.chx_serve_open, (None, 64, 1, 256, 1, 0, ctypes.byref(sid))),
+                     (L.chx_serve_submit, (None, 1, 0x1000, None)),
+                     (L.chx_serve_submit_device, (None, 1, 0x1000, None)),
+                     (L.chx_serve_wait_frame, (None, 1)),
+                     (L.chx_serve_read, (None, 1, None)),
+                     (L.chx_serve_close, (None, 1)),
+                     (L.chx_serve_get_stats, (None, None)),
+                     (L.chx_serve_reset_stats, (None,)),
+                     (L.chx_serve_latencies, (None, None, None, 0, None)),
+                     (L.chx_serve_destroy, (None,))):
+        assert fn(*args) == chx.ERR_INVALID_ARGUMENT, fn.__name__
