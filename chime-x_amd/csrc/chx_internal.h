@@ -49,7 +49,7 @@ int launch_blocks_to_triangle(int32_t F, int32_t N, const int32_t *blocks,
 
 // int8 matrix-core path (chx_mfma.hip).  mode: 0 assign, 1 add.
 // The plane workspace of one time chunk is at most kWorkspaceBytes.
-constexpr int64_t kWorkspaceBytes = 4LL << 30;
+constexpr int64_t kWorkspaceBytes = 16LL << 30;
 constexpr int kMfmaTile = 128;   // elements per output tile side
 constexpr int kMfmaStage = 64;   // samples per pipeline stage
 constexpr int kMfmaRing = 4;     // stages in flight; stage counts are padded to it
