@@ -34,8 +34,10 @@ def cmac(F, T, N):
 
 def cpu_baseline(budget_s: float = 12.0):
     """The reference CPU correlator (cpu_corr_test.c:156-203, generator
-    included, single thread) on one channel of the workload, T scaled so the
-    run takes about budget_s seconds.  Falls back to the C restatement."""
+    included) on channels of the workload, T scaled so a run takes about
+    budget_s / 2 seconds: once on one thread, then one channel per thread on
+    every usable host core (up to 16, the channels being independent).
+    Falls back to the C restatement when the reference is not built."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     ref = os.path.join(HERE, "oracle", "_ref", "libchxref.so")
     N = N_ELEM
@@ -63,17 +65,58 @@ def cpu_baseline(budget_s: float = 12.0):
         dt = time.perf_counter() - t0
         if rc != 0:
             return None
-        if dt * 2 > budget_s or T >= 4096:
+        if dt * 4 > budget_s or T >= 4096:
             break
-        T = min(4096, max(T * 2, int(T * budget_s / max(dt, 1e-3) / 1.5) // 8 * 8))
+        T = min(4096, max(T * 2, int(T * budget_s / 2 / max(dt, 1e-3) / 1.5) // 8 * 8))
+    one = cmac(1, T, N) / dt / 1e12
+    # one channel per thread (ctypes releases the GIL during the call)
+    import threading
+    cores = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+                       else (os.cpu_count() or 1)))
+    outs = [(ctypes.c_int32 * (N * (N + 1)))() for _ in range(cores)]
+    th = [threading.Thread(target=call, args=(T, outs[k])) for k in range(cores)]
+    t0 = time.perf_counter()
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    dtm = time.perf_counter() - t0
     return {
-        "value": cmac(1, T, N) / dt / 1e12,
+        "value": cores * cmac(1, T, N) / dtm / 1e12,
         "unit": "TCMAC/s",
-        "cores": 1,
+        "cores": cores,
         "kind": kind,
-        "sample": f"N={N}, F=1 (one channel of the workload), T={T} continuous seeded "
-                  f"random, generator included, {dt:.2f} s wall",
+        "single_thread_value": one,
+        "sample": f"N={N}, one channel of the workload per thread, T={T} continuous seeded "
+                  f"random, generator included: 1 thread {dt:.2f} s, {cores} threads "
+                  f"{dtm:.2f} s wall",
     }
+
+
+def traffic_from_profile():
+    """HBM bytes per GEMM launch measured with rocprofv3 PMC counters
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the same counters' reading for a
+    copy of known size), as recorded in the kernel's profile summary."""
+    import re
+    path = os.path.join(HERE, "profiles", "corr_mfma_N2048_F16_T65536.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
+
+
+def copy_calibration(torch, gib: int = 1, reps: int = 5):
+    """A plain device copy of known size, for calibrating PMC byte counters."""
+    n = gib << 30
+    a = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n, "reps": reps,
+                      "expected_fetch_bytes": n, "expected_write_bytes": n}))
 
 
 def main() -> int:
@@ -86,6 +129,8 @@ def main() -> int:
                     help="skip timing the plain packed-integer paths")
     ap.add_argument("--only", choices=["int8_mfma", "conference", "highly_packed"],
                     help="time one path only (for profiling)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -99,6 +144,9 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        copy_calibration(torch)
+        return 0
     dev = torch.device("cuda:0")
     F, T, N = N_FREQ, T_SAMPLES, N_ELEM
     g = torch.Generator(device=dev)
@@ -124,12 +172,25 @@ def main() -> int:
             per = [ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]
         return sum(per) / len(per), wall
 
+    def kernel_times(steps):
+        """Per-launch unpack and GEMM durations from HIP events the library
+        records on its stream around each kernel (a separate, untimed pass)."""
+        with chx.Plan(N, F, T, 256, 1, chx.INT8_MFMA) as p:
+            p.correlate_device(d_in.data_ptr(), out_mfma.data_ptr(), s)
+            torch.cuda.synchronize()
+            p.profile(True)
+            for _ in range(steps):
+                p.correlate_device(d_in.data_ptr(), out_mfma.data_ptr(), s)
+            u, g, n = p.kernel_times()
+        return u / n, g / n, n
+
     out_mfma = torch.empty((F, chx.num_blocks(N), 32, 32, 2), dtype=torch.int32, device=dev)
     result = {}
     only = args.only
     if only in (None, "int8_mfma"):
         ms, wall = time_path(chx.INT8_MFMA, 256, args.steps, args.warmup, out_mfma)
         result["int8_mfma"] = (ms, wall)
+        result["kernels"] = kernel_times(max(3, min(args.steps, 10)))
     packed = {}
     if not args.no_packed and only in (None, "conference", "highly_packed"):
         out_p = torch.empty_like(out_mfma)
@@ -150,8 +211,13 @@ def main() -> int:
         return 0
 
     ms, wall = result["int8_mfma"]
+    unpack_ms, gemm_ms, launches = result["kernels"]
     value = work / (ms * 1e-3) / 1e12
-    achieved_tops = 8 * work / (ms * 1e-3) / 1e12
+    step_tops = 8 * work / (ms * 1e-3) / 1e12
+    achieved_tops = 8 * work / (gemm_ms * 1e-3) / 1e12  # GEMM kernel alone
+    # unpack: reads the input once, writes 2 int8 planes (N and T are tile
+    # multiples here, so no padding)
+    unpack_bytes = T * F * N + 2 * T * F * N
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     line = {
         "metric": "complex MAC/s (Tera) at N=2048, 16 freq; % INT8 MFMA peak; bit-exact",
@@ -170,16 +236,26 @@ def main() -> int:
         "config": {"workload": "array-scale N=2048 F=16 T=65536", "N": N, "F": F, "T": T,
                    "method": "int8_mfma", "time_accum": 256,
                    "output": "block order, 16 x 2080 x 32 x 32 x (re,im) int32"},
-        "pct_int8_peak": round(100 * achieved_tops / INT8_PEAK_TOPS, 2),
+        "pct_int8_peak": round(100 * step_tops / INT8_PEAK_TOPS, 2),
         "wall_s_timed": round(wall, 4),
         "roofline": {
             "bound": "mfma",
+            "kernel": "corr_mfma_kernel (int8 GEMM pass)",
             "achieved": round(achieved_tops, 2),
             "peak": INT8_PEAK_TOPS,
             "unit": "TFLOP/s",
-            "ops": "int8 ops (1 MAC = 2 ops, 8 ops per complex MAC), algorithmic F*T*N(N+1)/2",
+            "ops": "int8 ops per launch = 8 * F*T*N(N+1)/2 (1 MAC = 2 ops, 4 real MACs per "
+                   "complex MAC), over the kernel's HIP-event duration",
             "frac": round(achieved_tops / INT8_PEAK_TOPS, 4),
-            "traffic": None,
+            "kernel_ms": round(gemm_ms, 4),
+            "launches_timed": launches,
+            "traffic": traffic_from_profile(),
+        },
+        "unpack_kernel": {
+            "bound": "hbm", "ms": round(unpack_ms, 4),
+            "achieved": round(unpack_bytes / (unpack_ms * 1e-3) / 1e9, 1), "peak": 8000.0,
+            "unit": "GB/s", "frac": round(unpack_bytes / (unpack_ms * 1e-3) / 8e12, 4),
+            "bytes": "input T*F*N read + 2 planes of T*F*N int8 written",
         },
         "packed_paths": packed,
         "speedup_vs_conference": (round(value / packed["conference"]["value"], 2)

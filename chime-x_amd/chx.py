@@ -143,6 +143,10 @@ def lib() -> ctypes.CDLL:
     for name in ("chx_offset_accumulate", "chx_preseed", "chx_corr", "chx_correlate"):
         getattr(L, name).argtypes = [vp, vp, vp, vp]
     L.chx_correlate_host.argtypes = [vp, vp, vp]
+    L.chx_plan_profile.argtypes = [vp, i32]
+    L.chx_plan_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_int64)]
     L.chx_num_blocks.argtypes = [i32]
     L.chx_num_blocks.restype = i64
     L.chx_block_maps.argtypes = [i32, vp, vp]
@@ -284,6 +288,17 @@ class Plan:
 
     def corr(self, d_input: int, d_vis: int, stream: int = 0) -> None:
         _check(lib().chx_corr(self._h, d_input, d_vis, stream or None), "chx_corr")
+
+    # -- kernel timing ------------------------------------------------------
+    def profile(self, enable: bool = True) -> None:
+        _check(lib().chx_plan_profile(self._h, 1 if enable else 0), "chx_plan_profile")
+
+    def kernel_times(self):
+        """(unpack ms, GEMM ms, chunk launches) summed since the last call."""
+        u, g, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        _check(lib().chx_plan_kernel_times(self._h, ctypes.byref(u), ctypes.byref(g),
+                                           ctypes.byref(n)), "chx_plan_kernel_times")
+        return u.value, g.value, n.value
 
 
 def blocks_to_triangle_device(num_freq: int, N: int, d_blocks: int, d_tri: int,

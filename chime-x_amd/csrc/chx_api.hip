@@ -5,6 +5,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <new>
+
 #include "chx_internal.h"
 
 namespace chx {
@@ -181,9 +183,51 @@ int chx_plan_create(const chx_config *cfg, chx_plan **out) {
   return CHX_SUCCESS;
 }
 
+int chx_plan_profile(chx_plan *p, int32_t enable) {
+  if (!p) return CHX_ERR_INVALID_ARGUMENT;
+  if (enable && !p->prof_ev) {
+    p->prof_ev = new (std::nothrow) std::vector<hipEvent_t>();
+    if (!p->prof_ev) return CHX_ERR_OUT_OF_MEMORY;
+  }
+  p->prof = enable ? 1 : 0;
+  return CHX_SUCCESS;
+}
+
+int chx_plan_kernel_times(chx_plan *p, double *unpack_ms, double *gemm_ms, int64_t *launches) {
+  if (!p) return CHX_ERR_INVALID_ARGUMENT;
+  double u = 0, g = 0;
+  int64_t n = 0;
+  int rc = CHX_SUCCESS;
+  if (p->prof_ev) {
+    std::vector<hipEvent_t> &ev = *p->prof_ev;
+    for (size_t k = 0; k + 2 < ev.size(); k += 3) {
+      float a = 0, b = 0;
+      hipError_t e = hipEventSynchronize(ev[k + 2]);
+      if (e == hipSuccess) e = hipEventElapsedTime(&a, ev[k], ev[k + 1]);
+      if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[k + 1], ev[k + 2]);
+      if (e != hipSuccess && !rc) rc = hip_status(e);
+      u += a;
+      g += b;
+      n++;
+    }
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+    ev.clear();
+  }
+  if (unpack_ms) *unpack_ms = u;
+  if (gemm_ms) *gemm_ms = g;
+  if (launches) *launches = n;
+  return rc;
+}
+
 int chx_plan_destroy(chx_plan *p) {
   if (!p) return CHX_ERR_INVALID_ARGUMENT;
   (void)hipSetDevice(p->cfg.device);
+  if (p->prof_ev) {
+    for (hipEvent_t e : *p->prof_ev)
+      if (e) (void)hipEventDestroy(e);
+    delete p->prof_ev;
+  }
   if (p->d_maps) (void)hipFree(p->d_maps);
   if (p->d_sums) (void)hipFree(p->d_sums);
   if (p->d_work) (void)hipFree(p->d_work);

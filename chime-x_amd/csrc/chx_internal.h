@@ -21,6 +21,10 @@ struct chx_plan {
   uint8_t *d_in;
   int32_t *d_out;
   size_t in_bytes, out_bytes, sums_bytes;
+  // Kernel timing (chx_plan_profile): event triples (before unpack, before
+  // GEMM, after GEMM) of each matrix-core chunk launch, not yet read.
+  int prof;
+  std::vector<hipEvent_t> *prof_ev;
 };
 
 namespace chx {

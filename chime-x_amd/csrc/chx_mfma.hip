@@ -427,8 +427,15 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
     // stages, padded to a multiple of the ring (the padding samples are 0)
     const int nst_chunk = (int)((tv + kStage * kRing - 1) / (kStage * kRing)) * kRing;
     const int KB = nst_chunk * (kStage / 16);
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    if (p->prof) {
+      for (hipEvent_t &e : ev) CHX_HIP(hipEventCreate(&e));
+      p->prof_ev->insert(p->prof_ev->end(), ev, ev + 3);
+      CHX_HIP(hipEventRecord(ev[0], s));
+    }
     int rc0 = launch_unpack_planes(in, p->d_work, N, F, Np, row_bytes, t0, tv, KB, s);
     if (rc0) return rc0;
+    if (p->prof) CHX_HIP(hipEventRecord(ev[1], s));
     // Split time only when the tiles alone cannot fill the chip twice over,
     // keeping >= 16 stages per split.
     int split = (int)((512 + items - 1) / items);
@@ -457,6 +464,7 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
     J.stages_per_split = nst_chunk / split;
     corr_mfma_kernel<<<dim3((unsigned)items, (unsigned)split), dim3(256), 0, s>>>(J);
     CHX_HIP(hipGetLastError());
+    if (p->prof) CHX_HIP(hipEventRecord(ev[2], s));
   }
   return CHX_SUCCESS;
 }

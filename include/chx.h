@@ -119,6 +119,16 @@ int chx_correlate(chx_plan *plan, const uint8_t *d_input, int32_t *d_vis,
  * buffer out; synchronous.  Device buffers are owned by the plan. */
 int chx_correlate_host(chx_plan *plan, const uint8_t *h_input, int32_t *h_vis);
 
+/* Kernel timing (no reference counterpart; for roofline measurements):
+ * while enabled, every int8 matrix-core chunk the plan launches records HIP
+ * events on its stream before the unpack kernel, between it and the GEMM
+ * kernel, and after the GEMM kernel.  chx_plan_kernel_times waits for the
+ * recorded launches, returns the summed unpack and GEMM durations and the
+ * number of chunk launches, and clears the records. */
+int chx_plan_profile(chx_plan *plan, int32_t enable);
+int chx_plan_kernel_times(chx_plan *plan, double *unpack_ms, double *gemm_ms,
+                          int64_t *launches);
+
 /* ---- Plan-owned device buffers and copies (for C hosts that do not link
  * HIP themselves; replace clCreateBuffer / clEnqueueWrite/ReadBuffer,
  * main_wrapper.c:379-501, 655-667, 798-799). ---- */
