@@ -7,10 +7,14 @@ a seed, device-resident input ([T][F][N] bytes, 2 GiB) and block-order int32
 output (16 x 2080 blocks, 260 MiB).  One step = one chx_correlate call (the
 whole integration of all 16 channels) through the C ABI.
 
-Prints ONE JSON line.  value = complex MACs per second (Tera) of the int8
-matrix-core path, algorithmic work F*T*N(N+1)/2 per step.  The plain
-packed-integer paths (conference, highly packed) are timed beside it, and
-the int8 result is compared bit-for-bit with the conference path's.
+Prints ONE JSON line.  Both matrix-core methods are timed: int8 operands
+with int32 accumulation, and FP6 E3M2 operands (every 4-bit value is exact)
+with FP32 accumulation kept below 2^24 (exact); their outputs are compared
+bit for bit.  value = complex MACs per second (Tera) of the faster one,
+algorithmic work F*T*N(N+1)/2 per step; its roofline is priced at its own
+dense peak (int8 5.0 POP/s, FP6 10.1 PFLOP/s).  The plain packed-integer
+paths (conference, highly packed) are timed beside them, and the conference
+output is compared bit for bit too.
 """
 from __future__ import annotations
 
@@ -26,6 +30,13 @@ sys.path.insert(0, os.path.join(HERE, "chime-x_amd"))
 
 N_ELEM, N_FREQ, T_SAMPLES = 2048, 16, 65536
 INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 matrix peak, 5.0 POP/s
+FP6_PEAK_TOPS = 10100.0  # MI355X dense FP6 matrix peak, 10.1 PFLOP/s
+MATRIX = {  # method name -> (kernel_batch, GEMM kernel, dtype, peak, plane bytes per value)
+    "int8_mfma": (2, "corr_mfma_kernel", "int8 operands, int32 accumulation", INT8_PEAK_TOPS,
+                  1.0),
+    "fp6_mfma": (3, "corr_f6_kernel", "fp6-e3m2 operands, fp32 accumulation (exact)",
+                 FP6_PEAK_TOPS, 0.75),
+}
 
 
 def cmac(F, T, N):
@@ -93,12 +104,12 @@ def cpu_baseline(budget_s: float = 12.0):
     }
 
 
-def traffic_from_profile():
+def traffic_from_profile(kernel: str):
     """HBM bytes per GEMM launch measured with rocprofv3 PMC counters
     (FETCH_SIZE + WRITE_SIZE, each scaled by the same counters' reading for a
     copy of known size), as recorded in the kernel's profile summary."""
     import re
-    path = os.path.join(HERE, "profiles", "corr_mfma_N2048_F16_T65536.md")
+    path = os.path.join(HERE, "profiles", f"{kernel.replace('_kernel', '')}_N2048_F16_T65536.md")
     try:
         text = open(path).read()
     except OSError:
@@ -127,7 +138,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-packed", action="store_true",
                     help="skip timing the plain packed-integer paths")
-    ap.add_argument("--only", choices=["int8_mfma", "conference", "highly_packed"],
+    ap.add_argument("--only", choices=["int8_mfma", "fp6_mfma", "conference", "highly_packed"],
                     help="time one path only (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (PMC byte-counter calibration)")
@@ -172,28 +183,60 @@ def main() -> int:
             per = [ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]
         return sum(per) / len(per), wall
 
-    def kernel_times(steps):
+    def kernel_times(method, steps, out):
         """Per-launch unpack and GEMM durations from HIP events the library
         records on its stream around each kernel (a separate, untimed pass)."""
-        with chx.Plan(N, F, T, 256, 1, chx.INT8_MFMA) as p:
-            p.correlate_device(d_in.data_ptr(), out_mfma.data_ptr(), s)
+        with chx.Plan(N, F, T, 256, 1, method) as p:
+            p.correlate_device(d_in.data_ptr(), out.data_ptr(), s)
             torch.cuda.synchronize()
             p.profile(True)
             for _ in range(steps):
-                p.correlate_device(d_in.data_ptr(), out_mfma.data_ptr(), s)
+                p.correlate_device(d_in.data_ptr(), out.data_ptr(), s)
             u, g, n = p.kernel_times()
         return u / n, g / n, n
 
-    out_mfma = torch.empty((F, chx.num_blocks(N), 32, 32, 2), dtype=torch.int32, device=dev)
-    result = {}
     only = args.only
-    if only in (None, "int8_mfma"):
-        ms, wall = time_path(chx.INT8_MFMA, 256, args.steps, args.warmup, out_mfma)
-        result["int8_mfma"] = (ms, wall)
-        result["kernels"] = kernel_times(max(3, min(args.steps, 10)))
+    mat = {}
+    outs = {}
+    for name, (method, kernel, dtype, peak, pbytes) in MATRIX.items():
+        if only not in (None, name):
+            continue
+        out = torch.empty((F, chx.num_blocks(N), 32, 32, 2), dtype=torch.int32, device=dev)
+        ms, wall = time_path(method, 256, args.steps, args.warmup, out)
+        unpack_ms, gemm_ms, launches = kernel_times(method, max(3, min(args.steps, 10)), out)
+        outs[name] = out
+        achieved = 8 * work / (gemm_ms * 1e-3) / 1e12  # GEMM kernel alone
+        # unpack: reads the input once, writes 2 planes (N and T are tile
+        # multiples here, so no padding)
+        ubytes = T * F * N + 2 * pbytes * T * F * N
+        mat[name] = {
+            "value": round(work / (ms * 1e-3) / 1e12, 3), "ms_per_step": round(ms, 4),
+            "wall_s_timed": round(wall, 4), "dtype": dtype,
+            "pct_of_own_peak": round(100 * 8 * work / (ms * 1e-3) / 1e12 / peak, 2),
+            "pct_int8_peak": round(100 * 8 * work / (ms * 1e-3) / 1e12 / INT8_PEAK_TOPS, 2),
+            "roofline": {
+                "bound": "mfma", "kernel": kernel, "achieved": round(achieved, 2),
+                "peak": peak, "unit": "TFLOP/s",
+                "ops": "8 * F*T*N(N+1)/2 per launch (1 MAC = 2 ops, 4 real MACs per complex "
+                       "MAC), over the kernel's HIP-event duration",
+                "frac": round(achieved / peak, 4), "kernel_ms": round(gemm_ms, 4),
+                "launches_timed": launches, "traffic": traffic_from_profile(kernel),
+            },
+            "unpack_kernel": {
+                "bound": "hbm", "ms": round(unpack_ms, 4),
+                "achieved": round(ubytes / (unpack_ms * 1e-3) / 1e9, 1), "peak": 8000.0,
+                "unit": "GB/s", "frac": round(ubytes / (unpack_ms * 1e-3) / 8e12, 4),
+                "bytes": f"input T*F*N read + 2 planes of {pbytes:g} byte per value written",
+            },
+        }
+    if len(outs) == 2:
+        same = bool(torch.equal(outs["int8_mfma"], outs["fp6_mfma"]))
+        for name in mat:
+            mat[name]["bit_exact_vs_other_matrix_path"] = same
+    ref_out = next(iter(outs.values()), None)
     packed = {}
     if not args.no_packed and only in (None, "conference", "highly_packed"):
-        out_p = torch.empty_like(out_mfma)
+        out_p = torch.empty((F, chx.num_blocks(N), 32, 32, 2), dtype=torch.int32, device=dev)
         # The packed paths are ~20x slower: a few steps are enough.
         k = max(1, min(3, args.steps))
         for name, method, ta in (("conference", chx.CONFERENCE, 256),
@@ -203,62 +246,40 @@ def main() -> int:
             ms, _ = time_path(method, ta, k, 1, out_p)
             packed[name] = {"value": work / (ms * 1e-3) / 1e12, "ms_per_step": ms,
                             "time_accum": ta}
-            if name == "conference" and "int8_mfma" in result:
-                packed[name]["bit_exact_vs_int8_mfma"] = bool(torch.equal(out_p, out_mfma))
+            if ref_out is not None:
+                packed[name]["bit_exact_vs_matrix_paths"] = bool(torch.equal(out_p, ref_out))
         del out_p
-    if only is not None and only != "int8_mfma":
+    if not mat:
         print(json.dumps({"only": only, "packed": packed}))
         return 0
 
-    ms, wall = result["int8_mfma"]
-    unpack_ms, gemm_ms, launches = result["kernels"]
-    value = work / (ms * 1e-3) / 1e12
-    step_tops = 8 * work / (ms * 1e-3) / 1e12
-    achieved_tops = 8 * work / (gemm_ms * 1e-3) / 1e12  # GEMM kernel alone
-    # unpack: reads the input once, writes 2 int8 planes (N and T are tile
-    # multiples here, so no padding)
-    unpack_bytes = T * F * N + 2 * T * F * N
+    best = max(mat, key=lambda n: mat[n]["value"])
+    b = mat[best]
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     line = {
         "metric": "complex MAC/s (Tera) at N=2048, 16 freq; % INT8 MFMA peak; bit-exact",
-        "value": round(value, 3),
+        "value": b["value"],
         "unit": "TCMAC/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 4),
+        "ms_per_step": b["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int8",
+        "dtype": b["dtype"],
         "data": "synthetic: uniform random bytes (4-bit complex in [-8,7]) from seed 42, "
                 "device-resident",
         "config": {"workload": "array-scale N=2048 F=16 T=65536", "N": N, "F": F, "T": T,
-                   "method": "int8_mfma", "time_accum": 256,
+                   "method": best, "time_accum": 256,
                    "output": "block order, 16 x 2080 x 32 x 32 x (re,im) int32"},
-        "pct_int8_peak": round(100 * step_tops / INT8_PEAK_TOPS, 2),
-        "wall_s_timed": round(wall, 4),
-        "roofline": {
-            "bound": "mfma",
-            "kernel": "corr_mfma_kernel (int8 GEMM pass)",
-            "achieved": round(achieved_tops, 2),
-            "peak": INT8_PEAK_TOPS,
-            "unit": "TFLOP/s",
-            "ops": "int8 ops per launch = 8 * F*T*N(N+1)/2 (1 MAC = 2 ops, 4 real MACs per "
-                   "complex MAC), over the kernel's HIP-event duration",
-            "frac": round(achieved_tops / INT8_PEAK_TOPS, 4),
-            "kernel_ms": round(gemm_ms, 4),
-            "launches_timed": launches,
-            "traffic": traffic_from_profile(),
-        },
-        "unpack_kernel": {
-            "bound": "hbm", "ms": round(unpack_ms, 4),
-            "achieved": round(unpack_bytes / (unpack_ms * 1e-3) / 1e9, 1), "peak": 8000.0,
-            "unit": "GB/s", "frac": round(unpack_bytes / (unpack_ms * 1e-3) / 8e12, 4),
-            "bytes": "input T*F*N read + 2 planes of T*F*N int8 written",
-        },
+        "pct_int8_peak": (mat["int8_mfma"]["pct_int8_peak"] if "int8_mfma" in mat else None),
+        "wall_s_timed": b["wall_s_timed"],
+        "roofline": b["roofline"],
+        "unpack_kernel": b["unpack_kernel"],
+        "matrix_paths": mat,
         "packed_paths": packed,
-        "speedup_vs_conference": (round(value / packed["conference"]["value"], 2)
+        "speedup_vs_conference": (round(b["value"] / packed["conference"]["value"], 2)
                                   if "conference" in packed else None),
         "cpu_baseline": cpu,
     }

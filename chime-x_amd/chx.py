@@ -29,8 +29,9 @@ REPO = os.path.dirname(HERE)
 LIB_PATH = os.environ.get("CHX_LIB", os.path.join(HERE, "libchx.so"))
 HEADER = os.path.join(REPO, "include", "chx.h")
 
-CONFERENCE, HIGHLY_PACKED, INT8_MFMA = 0, 1, 2
-METHOD_NAMES = {CONFERENCE: "conference", HIGHLY_PACKED: "highly_packed", INT8_MFMA: "int8_mfma"}
+CONFERENCE, HIGHLY_PACKED, INT8_MFMA, FP6_MFMA = 0, 1, 2, 3
+METHOD_NAMES = {CONFERENCE: "conference", HIGHLY_PACKED: "highly_packed", INT8_MFMA: "int8_mfma",
+                FP6_MFMA: "fp6_mfma"}
 
 SUCCESS = 0
 ERR_INVALID_ARGUMENT, ERR_SHAPE, ERR_TIME_ACCUM, ERR_NO_DEVICE = 1, 2, 3, 4

@@ -25,7 +25,7 @@ int hip_status(hipError_t e) {
 // steps), enforced here because the reference does not check them.
 static int validate(const chx_config *c) {
   if (!c) return CHX_ERR_INVALID_ARGUMENT;
-  if (c->kernel_batch < CHX_METHOD_CONFERENCE || c->kernel_batch > CHX_METHOD_INT8_MFMA)
+  if (c->kernel_batch < CHX_METHOD_CONFERENCE || c->kernel_batch > CHX_METHOD_FP6_MFMA)
     return CHX_ERR_INVALID_ARGUMENT;
   if (c->upper_triangle_convention != 0 && c->upper_triangle_convention != 1)
     return CHX_ERR_INVALID_ARGUMENT;
@@ -76,6 +76,7 @@ int32_t chx_max_time_accum(int32_t kernel_batch) {
     case CHX_METHOD_CONFERENCE: return CHX_CONFERENCE_MAX_TIME_ACCUM;
     case CHX_METHOD_HIGHLY_PACKED: return CHX_HIGHLY_PACKED_MAX_TIME_ACCUM;
     case CHX_METHOD_INT8_MFMA: return 0;
+    case CHX_METHOD_FP6_MFMA: return 0;
     default: return -1;
   }
 }
@@ -168,10 +169,12 @@ int chx_plan_create(const chx_config *cfg, chx_plan **out) {
     e = hipMemcpy(p->d_maps, maps, sizeof(uint32_t) * 2 * p->NB, hipMemcpyHostToDevice);
   free(maps);
   p->ntiles = (cfg->num_elements + 127) / 128;
-  if (e == hipSuccess && cfg->kernel_batch != CHX_METHOD_INT8_MFMA)
-    e = hipMalloc((void **)&p->d_sums, p->sums_bytes);
-  if (e == hipSuccess && cfg->kernel_batch == CHX_METHOD_INT8_MFMA) {
-    p->work_bytes = mfma_workspace_bytes(p);
+  const bool matrix = cfg->kernel_batch == CHX_METHOD_INT8_MFMA ||
+                      cfg->kernel_batch == CHX_METHOD_FP6_MFMA;
+  if (e == hipSuccess && !matrix) e = hipMalloc((void **)&p->d_sums, p->sums_bytes);
+  if (e == hipSuccess && matrix) {
+    p->work_bytes = cfg->kernel_batch == CHX_METHOD_FP6_MFMA ? fp6_workspace_bytes(p)
+                                                             : mfma_workspace_bytes(p);
     e = hipMalloc((void **)&p->d_work, p->work_bytes);
   }
   if (e != hipSuccess) {
@@ -259,7 +262,7 @@ int chx_offset_accumulate(const chx_plan *p, const uint8_t *d_input, uint32_t *d
 
 int chx_preseed(const chx_plan *p, const uint32_t *d_sums, int32_t *d_vis, void *stream) {
   if (!p || !d_sums || !d_vis) return CHX_ERR_INVALID_ARGUMENT;
-  if (p->cfg.kernel_batch == CHX_METHOD_INT8_MFMA) return CHX_ERR_INVALID_ARGUMENT;
+  if (p->cfg.kernel_batch >= CHX_METHOD_INT8_MFMA) return CHX_ERR_INVALID_ARGUMENT;
   return launch_preseed(p, d_sums, d_vis, (hipStream_t)stream);
 }
 
@@ -267,6 +270,8 @@ int chx_corr(const chx_plan *p, const uint8_t *d_input, int32_t *d_vis, void *st
   if (!p || !d_input || !d_vis) return CHX_ERR_INVALID_ARGUMENT;
   if (p->cfg.kernel_batch == CHX_METHOD_INT8_MFMA)
     return launch_corr_mfma(p, d_input, d_vis, 1, (hipStream_t)stream);
+  if (p->cfg.kernel_batch == CHX_METHOD_FP6_MFMA)
+    return launch_corr_fp6(p, d_input, d_vis, 1, (hipStream_t)stream);
   return launch_corr_packed(p, d_input, d_vis, (hipStream_t)stream);
 }
 
@@ -275,6 +280,7 @@ int chx_correlate(chx_plan *p, const uint8_t *d_input, int32_t *d_vis, void *str
   hipStream_t s = (hipStream_t)stream;
   if (p->cfg.kernel_batch == CHX_METHOD_INT8_MFMA)
     return launch_corr_mfma(p, d_input, d_vis, 0, s);
+  if (p->cfg.kernel_batch == CHX_METHOD_FP6_MFMA) return launch_corr_fp6(p, d_input, d_vis, 0, s);
   int rc = launch_fill_u32(p->d_sums, 0u, (int64_t)p->sums_bytes / 4, s);
   if (!rc) rc = launch_offset_accumulate(p, d_input, p->d_sums, s);
   if (!rc) rc = launch_preseed(p, p->d_sums, d_vis, s);

@@ -1,0 +1,438 @@
+// chx_fp6.hip -- the correlator on CDNA4 FP6 matrix cores, exactly.
+//
+// The 4-bit values v = nibble - 8 in [-8, 7] are all exact in the FP6
+// E3M2 format (1 sign, 3 exponent bits with bias 3, 2 mantissa bits):
+// 0 -> 0, 1..8 -> 2^e (1 + m/4).  v_mfma_scale_f32_32x32x64_f8f6f4 with
+// both operands E3M2 (and block scales 2^0) issues a 32x32x64 product in the
+// cycles an int8 32x32x32 one takes, and accumulates in FP32.  Every product
+// is an integer of magnitude <= 64 and every partial sum an integer; an
+// accumulator holds at most kChunkStages * 64 samples, |sum| <= 64 * 32640 <
+// 2^21, so all sums are exact in FP32 (24-bit significand) and the flushed
+// int32 visibilities are bit-identical to the int8 path's.  The same four
+// products as the int8 path per 32x32 block:
+//   RE = Ar.Br^T + Ai.Bi^T,  IX = Ai.Br^T,  IY = Ar.Bi^T,  im = +-(IX - IY).
+//
+// Two passes per time chunk:
+//  1. unpack_f6_kernel (HBM-bound): [time][freq][element] bytes -> E3M2
+//     codes, bit-packed 32 per 24 bytes (code j at bits 6j..6j+5, little
+//     endian), as two arrays per plane: P0 = bytes 0-15 and P1 = bytes 16-23
+//     of each element's 32-sample half of a 64-sample block:
+//       P0[plane][F][KB][half][Np][16 B],  P1[plane][F][KB][half][Np][8 B].
+//     A lane's MFMA operand (row l%32, samples 32(l/32)..+31 of the block)
+//     is then one 16-byte and one 8-byte LDS read at lane * 16 / lane * 8.
+//  2. corr_f6_kernel (matrix-core-bound): a workgroup = 4 waves = a
+//     128x128-element tile of one channel, each wave a 64x64 quarter (2x2
+//     blocks x 4 FP32 accumulators = 256 AGPRs), fed by buffer->LDS DMA
+//     through a ring of 6 stages of one 64-sample k-step (24 KB each).
+#include "chx_internal.h"
+
+namespace chx {
+namespace {
+
+typedef int v2i __attribute__((ext_vector_type(2)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) void lds_void;
+
+constexpr int kTile = kMfmaTile;     // elements per tile side
+constexpr int kK = 64;               // samples per k-step (= per stage)
+constexpr int kRing = kFp6Ring;      // LDS stages in the ring
+constexpr int kChunkStages = 510;    // stages per accumulator flush (multiple of kRing)
+constexpr int kP0Bytes = 16 * 1024;  // [plane 2][group 8][64 lanes][16 B]
+constexpr int kP1Bytes = 8 * 1024;   // [plane 2][group 8][64 lanes][8 B]
+constexpr int kSlotBytes = kP0Bytes + kP1Bytes;
+static_assert(kChunkStages % kRing == 0, "flush chunks hold whole rings");
+
+// Four nibbles (one per byte, 0-15) -> the E3M2 codes of nibble - 8 (one per
+// byte), by a 16-entry byte table in four words: entries 0-7 (v = -8..-1)
+// in the first pair, 8-15 (v = 0..7) in the second.
+__device__ __forceinline__ uint32_t e3m2_codes(uint32_t n) {
+  const uint32_t sel = n & 0x07070707u;
+  const uint32_t neg = __builtin_amdgcn_perm(0x2C303234u, 0x35363738u, sel);
+  const uint32_t pos = __builtin_amdgcn_perm(0x17161514u, 0x12100C00u, sel);
+  const uint32_t m = ((n >> 3) & 0x01010101u) * 0xFFu;  // 0xFF where nibble >= 8
+  return (pos & m) | (neg & ~m);
+}
+
+// Four 6-bit codes (one per byte) -> 24 contiguous bits.
+__device__ __forceinline__ uint32_t pack24(uint32_t c) {
+  return (c & 0x3Fu) | ((c >> 2) & 0xFC0u) | ((c >> 4) & 0x3F000u) | ((c >> 6) & 0xFC0000u);
+}
+
+// 32 codes (c[k] = samples 4k..4k+3) -> 24 bytes: P0 = o[0..3], P1 = o[4..5].
+__device__ __forceinline__ void pack32(const uint32_t *c, v4u &p0, v2u &p1) {
+  uint32_t g[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) g[k] = pack24(c[k]);
+  p0 = (v4u){g[0] | (g[1] << 24), (g[1] >> 8) | (g[2] << 16), (g[2] >> 16) | (g[3] << 8),
+             g[4] | (g[5] << 24)};
+  p1 = (v2u){(g[5] >> 8) | (g[6] << 16), (g[6] >> 16) | (g[7] << 8)};
+}
+
+// 4x4 byte transpose: r[s] holds byte e of sample s; c[e] gets byte s of
+// element e (v_perm_b32 selects bytes of {s0:s1}: 0-3 = s1, 4-7 = s0).
+__device__ __forceinline__ void transpose4(uint32_t r0, uint32_t r1, uint32_t r2,
+                                           uint32_t r3, uint32_t &c0, uint32_t &c1,
+                                           uint32_t &c2, uint32_t &c3) {
+  const uint32_t t01l = __builtin_amdgcn_perm(r1, r0, 0x05010400u);
+  const uint32_t t01h = __builtin_amdgcn_perm(r1, r0, 0x07030602u);
+  const uint32_t t23l = __builtin_amdgcn_perm(r3, r2, 0x05010400u);
+  const uint32_t t23h = __builtin_amdgcn_perm(r3, r2, 0x07030602u);
+  c0 = __builtin_amdgcn_perm(t23l, t01l, 0x05040100u);
+  c1 = __builtin_amdgcn_perm(t23l, t01l, 0x07060302u);
+  c2 = __builtin_amdgcn_perm(t23h, t01h, 0x05040100u);
+  c3 = __builtin_amdgcn_perm(t23h, t01h, 0x07060302u);
+}
+
+// Pass 1.  Workgroup = 256 element quads (1024 elements) x one 64-sample
+// block of one channel.  Thread: 64 word loads (one per sample row; 256
+// contiguous bytes per wave and row), 16 byte transposes, then per plane:
+// codes, bit packing, LDS staging, and full-run global stores (1 KB of P0 or
+// 512 B of P1 per wave instruction).
+__global__ __launch_bounds__(256) void unpack_f6_kernel(
+    const uint8_t *__restrict__ in, uint8_t *__restrict__ p0, uint8_t *__restrict__ p1, int N,
+    int F, int Np, int64_t row_bytes, int64_t t0, int64_t t_valid, int KB) {
+  __shared__ __attribute__((aligned(16))) v4u s0[2][1024 + 64];  // [half][element] P0, padded
+  __shared__ __attribute__((aligned(16))) v2u s1[2][1024 + 64];  // [half][element] P1, padded
+  const int tid = threadIdx.x;
+  const int q = blockIdx.x * 256 + tid;  // element quad
+  const int kb = blockIdx.y, f = blockIdx.z;
+  const bool evalid = 4 * q < N;
+  uint32_t c[4][16];  // [element][4-sample group]
+  {
+    const uint8_t *src = in + (t0 + (int64_t)kK * kb) * row_bytes + (int64_t)f * N + 4 * q;
+    uint32_t w[kK];
+#pragma unroll
+    for (int s = 0; s < kK; s++) {
+      const bool ok = evalid && ((int64_t)kK * kb + s < t_valid);
+      w[s] = ok ? *(const uint32_t *)(src + s * row_bytes) : 0x88888888u;  // 0x88 -> code 0
+    }
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      transpose4(w[4 * m], w[4 * m + 1], w[4 * m + 2], w[4 * m + 3], c[0][m], c[1][m], c[2][m],
+                 c[3][m]);
+  }
+  const int e_base = blockIdx.x * 1024;
+  const int n_el = Np - e_base < 1024 ? Np - e_base : 1024;
+#pragma unroll
+  for (int pl = 0; pl < 2; pl++) {
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      uint32_t code[16];
+#pragma unroll
+      for (int m = 0; m < 16; m++)
+        code[m] = e3m2_codes(pl == 0 ? (c[e][m] >> 4) & 0x0F0F0F0Fu : c[e][m] & 0x0F0F0F0Fu);
+      const int el = 4 * tid + e;
+      const int slot = el + (el >> 4);  // one pad entry per 16
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        v4u a;
+        v2u b;
+        pack32(&code[8 * h], a, b);
+        s0[h][slot] = a;
+        s1[h][slot] = b;
+      }
+    }
+    __syncthreads();
+    const int64_t blk = (((int64_t)pl * F + f) * KB + kb) * 2;  // (plane, channel, block) half 0
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      uint8_t *d0 = p0 + ((blk + h) * Np + e_base) * 16;
+      uint8_t *d1 = p1 + ((blk + h) * Np + e_base) * 8;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int el = k * 256 + tid;
+        if (el < n_el) {
+          const int slot = el + (el >> 4);
+          *(v4u *)(d0 + el * 16) = s0[h][slot];
+          *(v2u *)(d1 + el * 8) = s1[h][slot];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Flush one block's FP32 accumulators (exact integers) into int32
+// visibilities.  Lane l owns column n = l%32 and rows m = 8*(j/4) +
+// 4*(l/32) + j%4 of register j.  mode 0 assigns, 1 adds, 2 adds atomically.
+__device__ __forceinline__ void flush_block_f32(int32_t *blk, const v16f &rea, const v16f &reb,
+                                                const v16f &ix, const v16f &iy, int U,
+                                                int mode, int lane) {
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc(blk, (short)0, 2048 * 4, 0x00020000);
+  const int voff = ((lane >> 5) * 4 * 32 + (lane & 31)) * 8;
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    const int32_t vr = (int32_t)(rea[j] + reb[j]);  // |.| < 2^22: exact
+    const int32_t vi = (int32_t)(U ? ix[j] - iy[j] : iy[j] - ix[j]);
+    const int soff = (8 * (j >> 2) + (j & 3)) * 32 * 8;
+    if (mode == 0) {
+      v2i v = {vr, vi};
+      __builtin_amdgcn_raw_buffer_store_b64(v, rsrc, voff, soff, 0);
+    } else if (mode == 1) {
+      v2i a = __builtin_amdgcn_raw_buffer_load_b64(rsrc, voff, soff, 0);
+      a.x += vr;
+      a.y += vi;
+      __builtin_amdgcn_raw_buffer_store_b64(a, rsrc, voff, soff, 0);
+    } else {
+      __builtin_amdgcn_raw_ptr_buffer_atomic_add_i32(vr, rsrc, voff, soff, 0);
+      __builtin_amdgcn_raw_ptr_buffer_atomic_add_i32(vi, rsrc, voff + 4, soff, 0);
+    }
+  }
+}
+
+__device__ __forceinline__ int64_t xcd_remap(int64_t id, int64_t total) {
+  const int64_t q = total / 8, r = total % 8;
+  const int64_t xcd = id % 8, slot = id / 8;
+  return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
+__device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti, int &tj) {
+  int r = 0;
+  while (tb >= ntiles - r) {
+    tb -= ntiles - r;
+    r++;
+  }
+  ti = r;
+  tj = r + tb;
+}
+
+// Pass 2, one output tile (ti, tj) of channel f over k-steps
+// [st_begin, st_begin + nst) (nst a multiple of kRing).
+__device__ __forceinline__ void gemm_tile_f6(const Fp6Job &J, const int f, const int ti,
+                                             const int tj, const int st_begin, const int nst,
+                                             uint8_t (*lds)[kSlotBytes]) {
+  const int F = J.F, Np = J.Np, KB = J.KB, nb = J.nb, U = J.U, mode = J.mode;
+  const int64_t NB = J.NB;
+  int32_t *vis = J.vis;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+
+  // Loads: wave w fills plane w/2, groups 4(w&1)..+3: four P0 operands and
+  // two P1 group pairs per stage.  One descriptor per (array, plane,
+  // channel); the lane offset picks element and half, the scalar offset the
+  // k-step.
+  const int lp = wave >> 1;
+  const int64_t p0_blk = (int64_t)2 * Np * 16;  // one 64-sample block of one channel
+  const int64_t p1_blk = (int64_t)2 * Np * 8;
+  const __amdgpu_buffer_rsrc_t r0 = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(J.p0 + ((int64_t)lp * F + f) * KB * p0_blk), (short)0, (int)(KB * p0_blk),
+      0x00020000);
+  const __amdgpu_buffer_rsrc_t r1 = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(J.p1 + ((int64_t)lp * F + f) * KB * p1_blk), (short)0, (int)(KB * p1_blk),
+      0x00020000);
+  int voff0[4], voff1[2];
+#pragma unroll
+  for (int gg = 0; gg < 4; gg++) {
+    const int g = 4 * (wave & 1) + gg;
+    const int e0 = (g < 4 ? ti : tj) * kTile + 32 * (g & 3);
+    voff0[gg] = ((lane >> 5) * Np + e0 + (lane & 31)) * 16;
+  }
+#pragma unroll
+  for (int pp = 0; pp < 2; pp++) {
+    const int g = 4 * (wave & 1) + 2 * pp + (lane >> 5);  // lanes 32-63: the pair's second group
+    const int e0 = (g < 4 ? ti : tj) * kTile + 32 * (g & 3);
+    const int i = lane & 31;
+    voff1[pp] = ((i >> 4) * Np + e0 + 2 * (i & 15)) * 8;
+  }
+  auto issue_stage = [&](int st, int slot) {
+    const int kb = st_begin + st;
+#pragma unroll
+    for (int gg = 0; gg < 4; gg++) {
+      const int g = 4 * (wave & 1) + gg;
+      lds_void *dst = (lds_void *)&lds[slot][(lp * 8 + g) * 1024];
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(r0, dst, 16, voff0[gg], kb * (int)p0_blk, 0, 0);
+    }
+#pragma unroll
+    for (int pp = 0; pp < 2; pp++) {
+      const int g = 4 * (wave & 1) + 2 * pp;
+      lds_void *dst = (lds_void *)&lds[slot][kP0Bytes + (lp * 8 + g) * 512];
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(r1, dst, 16, voff1[pp], kb * (int)p1_blk, 0, 0);
+    }
+  };
+
+  const int wr = wave >> 1, wc = wave & 1;
+  v16f rea[2][2], reb[2][2], acc_ix[2][2], acc_iy[2][2];
+  const v16f zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) rea[i][j] = reb[i][j] = acc_ix[i][j] = acc_iy[i][j] = zero;
+  struct Ops {
+    v8i ar[2], ai[2], br[2], bi[2];
+  };
+  auto load_op = [&](int slot, int plane, int g) -> v8i {
+    const v4i a = *(const v4i *)&lds[slot][(plane * 8 + g) * 1024 + lane * 16];
+    const v2i b = *(const v2i *)&lds[slot][kP0Bytes + (plane * 8 + g) * 512 + lane * 8];
+    return (v8i){a.x, a.y, a.z, a.w, b.x, b.y, 0, 0};
+  };
+  auto read_ops = [&](Ops &o, int slot) {
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const int ga = 2 * wr + i, gb = 4 + 2 * wc + i;
+      o.ar[i] = load_op(slot, 0, ga);
+      o.br[i] = load_op(slot, 0, gb);
+      o.ai[i] = load_op(slot, 1, ga);
+      o.bi[i] = load_op(slot, 1, gb);
+    }
+  };
+  auto mfma_ops = [&](const Ops &o) {
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        rea[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(o.ar[i], o.br[j], rea[i][j],
+                                                                    3, 3, 0, 127, 0, 127);
+        acc_iy[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+            o.ar[i], o.bi[j], acc_iy[i][j], 3, 3, 0, 127, 0, 127);
+        acc_ix[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+            o.ai[i], o.br[j], acc_ix[i][j], 3, 3, 0, 127, 0, 127);
+        reb[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(o.ai[i], o.bi[j], reb[i][j],
+                                                                    3, 3, 0, 127, 0, 127);
+      }
+  };
+  int cur_mode = mode;
+  const int by0 = (ti * kTile + 64 * wr) / 32, bx0 = (tj * kTile + 64 * wc) / 32;
+
+  // Prologue: stages 0..kRing-2 in flight; wait for this wave's stage 0
+  // (6 loads per stage, so (kRing-2)*6 may stay pending), barrier, read it.
+#pragma unroll
+  for (int s = 0; s < kRing - 1; s++) issue_stage(s < nst ? s : nst - 1, s);
+  asm volatile("s_waitcnt vmcnt(24)\n\ts_barrier" ::: "memory");
+  Ops o[2];
+  read_ops(o[0], 0);
+
+  // Stage st (slot st % kRing): refill the slot of stage st-1 with stage
+  // st+kRing-1; wait for this wave's loads of st+1, barrier (all waves' loads
+  // landed, all reads of the refilled slot done); read st+1's operands while
+  // st's MFMAs run.
+  for (int c0 = 0; c0 < nst; c0 += kChunkStages) {
+    const int c1 = c0 + kChunkStages < nst ? c0 + kChunkStages : nst;
+    for (int st0 = c0; st0 < c1; st0 += kRing) {
+#pragma unroll
+      for (int d = 0; d < kRing; d++) {
+        const int st = st0 + d;
+        const int nx = st + kRing - 1;
+        issue_stage(nx < nst ? nx : nst - 1, (d + kRing - 1) % kRing);
+        asm volatile("s_waitcnt vmcnt(24)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        read_ops(o[(d + 1) & 1], (d + 1) % kRing);
+        mfma_ops(o[d & 1]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const int by = by0 + i, bx = bx0 + j;
+        if (bx >= by && bx < nb) {
+          const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
+          flush_block_f32(vis + ((int64_t)f * NB + b) * 2048, rea[i][j], reb[i][j],
+                          acc_ix[i][j], acc_iy[i][j], U, cur_mode, lane);
+        }
+        rea[i][j] = reb[i][j] = acc_ix[i][j] = acc_iy[i][j] = zero;
+      }
+    if (cur_mode == 0) cur_mode = 1;
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+// One plan: grid.x = channels x upper-triangle tiles (XCD-remapped), grid.y
+// = time splits.
+__global__ __launch_bounds__(256, 1) void corr_f6_kernel(Fp6Job J) {
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kSlotBytes];
+  const int64_t item = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
+  const int64_t per_f = (int64_t)J.ntiles * (J.ntiles + 1) / 2;
+  int ti, tj;
+  tile_coords((int)(item % per_f), J.ntiles, ti, tj);
+  gemm_tile_f6(J, (int)(item / per_f), ti, tj, blockIdx.y * J.stages_per_split,
+               J.stages_per_split, lds);
+}
+
+}  // namespace
+
+// Samples per chunk: planes of one chunk within kWorkspaceBytes, a
+// multiple of the ring (kRing k-steps), one (array, plane, channel) region
+// addressable by a buffer descriptor.
+static int64_t f6_chunk_samples(const chx_plan *p) {
+  const int64_t Np = (int64_t)p->ntiles * kTile;
+  const int64_t unit = (int64_t)kK * kRing;
+  const int64_t per_sample = 2 * (int64_t)p->cfg.num_freq * Np * 3 / 4;  // 6 bits per value
+  int64_t tc = kWorkspaceBytes / per_sample;
+  const int64_t desc = 0x7FFFFFFFLL / (Np / 2);  // KB * 2 * Np * 16 = tc * Np / 2 bytes
+  if (tc > desc) tc = desc;
+  tc = tc / unit * unit;
+  const int64_t T = (p->cfg.time_steps + unit - 1) / unit * unit;
+  if (tc > T) tc = T;
+  if (tc < unit) tc = unit;
+  return tc;
+}
+
+size_t fp6_workspace_bytes(const chx_plan *p) {
+  const int64_t Np = (int64_t)p->ntiles * kTile;
+  return (size_t)(2 * (int64_t)p->cfg.num_freq * Np * f6_chunk_samples(p) * 3 / 4);
+}
+
+int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode,
+                    hipStream_t s) {
+  const int N = p->cfg.num_elements, F = p->cfg.num_freq, U = p->cfg.upper_triangle_convention;
+  const int64_t T = p->cfg.time_steps;
+  const int ntiles = p->ntiles, Np = ntiles * kTile;
+  const int64_t items = (int64_t)F * ntiles * (ntiles + 1) / 2;
+  if (items > 0x7FFFFFFF || !p->d_work) return CHX_ERR_SHAPE;
+  const int64_t tc = f6_chunk_samples(p);
+  const int64_t row_bytes = (int64_t)F * N;
+  for (int64_t t0 = 0; t0 < T; t0 += tc) {
+    const int64_t tv = T - t0 < tc ? T - t0 : tc;
+    const int nst = (int)((tv + (int64_t)kK * kRing - 1) / ((int64_t)kK * kRing)) * kRing;
+    const int KB = nst;  // one 64-sample block per stage
+    uint8_t *p0 = p->d_work;
+    uint8_t *p1 = p0 + (int64_t)2 * F * KB * 2 * Np * 16;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    if (p->prof) {
+      for (hipEvent_t &e : ev) CHX_HIP(hipEventCreate(&e));
+      p->prof_ev->insert(p->prof_ev->end(), ev, ev + 3);
+      CHX_HIP(hipEventRecord(ev[0], s));
+    }
+    unpack_f6_kernel<<<dim3((unsigned)((Np + 1023) / 1024), (unsigned)KB, (unsigned)F),
+                       dim3(256), 0, s>>>(in, p0, p1, N, F, Np, row_bytes, t0, tv, KB);
+    CHX_HIP(hipGetLastError());
+    if (p->prof) CHX_HIP(hipEventRecord(ev[1], s));
+    // Split time only when the tiles alone cannot fill the chip twice over,
+    // keeping >= 16 stages per split; splits hold whole rings.
+    int split = (int)((512 + items - 1) / items);
+    if (split > nst / 16) split = nst / 16;
+    if (split < 1) split = 1;
+    while ((nst / kRing) % split) split--;
+    int kmode = (t0 == 0) ? mode : 1;
+    if (split > 1) {
+      if (kmode == 0) {
+        int rc = launch_fill_u32((uint32_t *)vis, 0u, (int64_t)F * p->NB * 2048, s);
+        if (rc) return rc;
+      }
+      kmode = 2;
+    }
+    Fp6Job J;
+    J.p0 = p0;
+    J.p1 = p1;
+    J.vis = vis;
+    J.NB = p->NB;
+    J.F = F;
+    J.Np = Np;
+    J.KB = KB;
+    J.ntiles = ntiles;
+    J.nb = p->nb;
+    J.U = U;
+    J.mode = kmode;
+    J.stages_per_split = nst / split;
+    corr_f6_kernel<<<dim3((unsigned)items, (unsigned)split), dim3(256), 0, s>>>(J);
+    CHX_HIP(hipGetLastError());
+    if (p->prof) CHX_HIP(hipEventRecord(ev[2], s));
+  }
+  return CHX_SUCCESS;
+}
+
+}  // namespace chx

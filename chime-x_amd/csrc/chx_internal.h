@@ -61,6 +61,19 @@ size_t mfma_workspace_bytes(const chx_plan *p);
 int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis,
                      int mode, hipStream_t s);
 
+// FP6 (E3M2) matrix-core path (chx_fp6.hip): same modes, bit-identical
+// results.
+constexpr int kFp6Ring = 6;  // 64-sample stages in the LDS ring
+size_t fp6_workspace_bytes(const chx_plan *p);
+int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode,
+                    hipStream_t s);
+struct Fp6Job {
+  const uint8_t *p0, *p1;  // E3M2 planes of the chunk: bytes 0-15 / 16-23 of each half
+  int32_t *vis;
+  int64_t NB;
+  int F, Np, KB, ntiles, nb, U, mode, stages_per_split;  // KB: 64-sample blocks
+};
+
 // One job as the matrix-core kernels see it (per time chunk).
 struct GemmJob {
   const uint8_t *planes;  // int8 planes [2][F][KB][Np][16] of the chunk

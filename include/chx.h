@@ -30,12 +30,15 @@ extern "C" {
 #endif
 
 /* kernel_batch values.  0 and 1 are the reference's -k values
- * (main_wrapper.c:281-305); 2 has no reference counterpart: the int8
- * matrix-core path, which needs no offset/preseed pass and has no
- * time_accum cap. */
+ * (main_wrapper.c:281-305); 2 and 3 have no reference counterpart: the
+ * matrix-core paths, which need no offset/preseed pass and have no
+ * time_accum cap.  2 computes on int8 operands with int32 accumulation; 3 on
+ * FP6 E3M2 operands (every 4-bit value is exact in E3M2) with FP32
+ * accumulation kept below 2^24, so its results are bit-identical to 2's. */
 #define CHX_METHOD_CONFERENCE 0
 #define CHX_METHOD_HIGHLY_PACKED 1
 #define CHX_METHOD_INT8_MFMA 2
+#define CHX_METHOD_FP6_MFMA 3
 
 /* time_accum caps of the packed methods: 16-bit lanes gain <= 225 per sample
  * (README.md:14; 288 is the largest multiple of 8 not above 291), and the

@@ -47,6 +47,7 @@ def test_strerror_and_caps():
     assert chx.lib().chx_max_time_accum(0) == 288
     assert chx.lib().chx_max_time_accum(1) == 2272
     assert chx.lib().chx_max_time_accum(2) == 0
+    assert chx.lib().chx_max_time_accum(3) == 0
     assert chx.lib().chx_max_time_accum(7) == -1
     assert b"gfx950" in chx.lib().chx_version()
 
@@ -73,7 +74,7 @@ def _create(**kw):
     (dict(time_steps=320, time_accum=256), chx.ERR_TIME_ACCUM),  # T % ta
     (dict(time_steps=296 * 4, time_accum=296), chx.ERR_TIME_ACCUM),  # conference cap
     (dict(kernel_batch=1, time_steps=2280 * 4, time_accum=2280), chx.ERR_TIME_ACCUM),
-    (dict(kernel_batch=3), chx.ERR_INVALID_ARGUMENT),
+    (dict(kernel_batch=4), chx.ERR_INVALID_ARGUMENT),
     (dict(upper_triangle_convention=2), chx.ERR_INVALID_ARGUMENT),
     (dict(num_freq=2 ** 20, num_elements=4096), chx.ERR_SHAPE),  # grid bound
 ])

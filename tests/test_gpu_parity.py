@@ -17,7 +17,8 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-METHODS = [chx.CONFERENCE, chx.HIGHLY_PACKED, chx.INT8_MFMA]
+METHODS = [chx.CONFERENCE, chx.HIGHLY_PACKED, chx.INT8_MFMA, chx.FP6_MFMA]
+MATRIX = [chx.INT8_MFMA, chx.FP6_MFMA]
 
 
 def say(*a):
@@ -82,7 +83,7 @@ def test_random_shapes_against_oracle(method, U):
 
 
 @pytest.mark.parametrize("method,ta", [(chx.CONFERENCE, 288), (chx.HIGHLY_PACKED, 2272),
-                                       (chx.INT8_MFMA, 2272)])
+                                       (chx.INT8_MFMA, 2272), (chx.FP6_MFMA, 2272)])
 def test_worst_case_lanes_at_cap(method, ta):
     T = ta * 2
     data = np.full((T, 1, 64), 0xFF, np.uint8)
@@ -129,11 +130,12 @@ def test_stage_operations(method, U):
         np.testing.assert_array_equal(d_vis.cpu().numpy(), O.packed_raw(data, method, U, ta))
 
 
-def test_mfma_corr_adds_and_device_triangle():
+@pytest.mark.parametrize("method", MATRIX)
+def test_mfma_corr_adds_and_device_triangle(method):
     torch = _torch()
     T, F, N = 256, 2, 128
     data = rand(T, F, N, 5)
-    with chx.Plan(N, F, T, 256, 1, chx.INT8_MFMA) as p:
+    with chx.Plan(N, F, T, 256, 1, method) as p:
         d_in = torch.from_numpy(data).cuda()
         d_vis = torch.full(p.output_shape, 7, dtype=torch.int32, device="cuda")
         p.corr(d_in.data_ptr(), d_vis.data_ptr())
@@ -147,25 +149,27 @@ def test_mfma_corr_adds_and_device_triangle():
         np.testing.assert_array_equal(d_tri.cpu().numpy(), O.blocks_to_triangle(want, N))
 
 
-def test_mfma_long_integration_saturation_closed_form():
-    # Every sample (-8,-8): re = 128*T, im = 0, over several 32768-sample
-    # accumulator chunks.
+@pytest.mark.parametrize("method", MATRIX)
+def test_mfma_long_integration_saturation_closed_form(method):
+    # Every sample (-8,-8): re = 128*T, im = 0, over several accumulator
+    # flush chunks (the largest accumulator values either path holds).
     T, F, N = 131072, 2, 128
     data = np.zeros((T, F, N), np.uint8)
-    blocks = run(data, chx.INT8_MFMA, 1, 256)
+    blocks = run(data, method, 1, 256)
     assert (blocks[..., 0] == 128 * T).all() and (blocks[..., 1] == 0).all()
 
 
-def test_mfma_time_split_and_linearity():
+@pytest.mark.parametrize("method", MATRIX)
+def test_mfma_time_split_and_linearity(method):
     # Small N with long T takes the split-time (atomic) path.
     T, F, N = 16384, 1, 64
     data = rand(T, F, N, 9)
-    full = run(data, chx.INT8_MFMA, 1, 256)
-    a = run(np.ascontiguousarray(data[:8192]), chx.INT8_MFMA, 1, 256)
-    b = run(np.ascontiguousarray(data[8192:]), chx.INT8_MFMA, 1, 256)
+    full = run(data, method, 1, 256)
+    a = run(np.ascontiguousarray(data[:8192]), method, 1, 256)
+    b = run(np.ascontiguousarray(data[8192:]), method, 1, 256)
     np.testing.assert_array_equal(full, a + b)
     np.testing.assert_array_equal(full, O.correlate_blocks(data, 1))
-    conj = run(data, chx.INT8_MFMA, 0, 256)
+    conj = run(data, method, 0, 256)
     np.testing.assert_array_equal(conj[..., 0], full[..., 0])
     np.testing.assert_array_equal(conj[..., 1], -full[..., 1])
 
@@ -176,6 +180,7 @@ def test_methods_agree_on_array_scale_channel_slice():
     T, F, N = 512, 2, 2048
     data = rand(T, F, N, 2048)
     want = O.correlate_blocks(data, 1)
-    for method, ta in ((chx.INT8_MFMA, 512), (chx.CONFERENCE, 256), (chx.HIGHLY_PACKED, 512)):
+    for method, ta in ((chx.INT8_MFMA, 512), (chx.FP6_MFMA, 512), (chx.CONFERENCE, 256),
+                       (chx.HIGHLY_PACKED, 512)):
         np.testing.assert_array_equal(run(data, method, 1, ta), want,
                                       err_msg=chx.METHOD_NAMES[method])
