@@ -87,31 +87,32 @@ __device__ __forceinline__ void transpose4(uint32_t r0, uint32_t r1, uint32_t r2
   c3 = __builtin_amdgcn_perm(t23h, t01h, 0x07060302u);
 }
 
-// Pass 1.  Workgroup = 256 element quads (1024 elements) x one 64-sample
-// block of one channel.  Thread: 64 word loads (one per sample row; 256
-// contiguous bytes per wave and row), 16 byte transposes, then per plane:
-// codes, bit packing, LDS staging, and full-run global stores (1 KB of P0 or
-// 512 B of P1 per wave instruction).
+// Pass 1.  Workgroup = 256 element quads (1024 elements) x one 32-sample
+// half of a 64-sample block of one channel.  Thread: 32 word loads (one per
+// sample row; 256 contiguous bytes per wave and row), 8 byte transposes,
+// then per plane: codes, bit packing, LDS staging, and full-run global
+// stores (1 KB of P0 or 512 B of P1 per wave instruction).
 __global__ __launch_bounds__(256) void unpack_f6_kernel(
     const uint8_t *__restrict__ in, uint8_t *__restrict__ p0, uint8_t *__restrict__ p1, int N,
     int F, int Np, int64_t row_bytes, int64_t t0, int64_t t_valid, int KB) {
-  __shared__ __attribute__((aligned(16))) v4u s0[2][1024 + 64];  // [half][element] P0, padded
-  __shared__ __attribute__((aligned(16))) v2u s1[2][1024 + 64];  // [half][element] P1, padded
+  __shared__ __attribute__((aligned(16))) v4u s0[1024 + 64];  // [element] P0, padded
+  __shared__ __attribute__((aligned(16))) v2u s1[1024 + 64];  // [element] P1, padded
   const int tid = threadIdx.x;
   const int q = blockIdx.x * 256 + tid;  // element quad
-  const int kb = blockIdx.y, f = blockIdx.z;
+  const int kb = blockIdx.y >> 1, h = blockIdx.y & 1, f = blockIdx.z;
   const bool evalid = 4 * q < N;
-  uint32_t c[4][16];  // [element][4-sample group]
+  const int64_t ts = (int64_t)kK * kb + 32 * h;  // first sample of the half
+  uint32_t c[4][8];  // [element][4-sample group]
   {
-    const uint8_t *src = in + (t0 + (int64_t)kK * kb) * row_bytes + (int64_t)f * N + 4 * q;
-    uint32_t w[kK];
+    const uint8_t *src = in + (t0 + ts) * row_bytes + (int64_t)f * N + 4 * q;
+    uint32_t w[32];
 #pragma unroll
-    for (int s = 0; s < kK; s++) {
-      const bool ok = evalid && ((int64_t)kK * kb + s < t_valid);
+    for (int s = 0; s < 32; s++) {
+      const bool ok = evalid && (ts + s < t_valid);
       w[s] = ok ? *(const uint32_t *)(src + s * row_bytes) : 0x88888888u;  // 0x88 -> code 0
     }
 #pragma unroll
-    for (int m = 0; m < 16; m++)
+    for (int m = 0; m < 8; m++)
       transpose4(w[4 * m], w[4 * m + 1], w[4 * m + 2], w[4 * m + 3], c[0][m], c[1][m], c[2][m],
                  c[3][m]);
   }
@@ -121,35 +122,29 @@ __global__ __launch_bounds__(256) void unpack_f6_kernel(
   for (int pl = 0; pl < 2; pl++) {
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-      uint32_t code[16];
+      uint32_t code[8];
 #pragma unroll
-      for (int m = 0; m < 16; m++)
+      for (int m = 0; m < 8; m++)
         code[m] = e3m2_codes(pl == 0 ? (c[e][m] >> 4) & 0x0F0F0F0Fu : c[e][m] & 0x0F0F0F0Fu);
       const int el = 4 * tid + e;
       const int slot = el + (el >> 4);  // one pad entry per 16
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        v4u a;
-        v2u b;
-        pack32(&code[8 * h], a, b);
-        s0[h][slot] = a;
-        s1[h][slot] = b;
-      }
+      v4u a;
+      v2u b;
+      pack32(code, a, b);
+      s0[slot] = a;
+      s1[slot] = b;
     }
     __syncthreads();
-    const int64_t blk = (((int64_t)pl * F + f) * KB + kb) * 2;  // (plane, channel, block) half 0
+    const int64_t row = (((int64_t)pl * F + f) * KB + kb) * 2 + h;  // (plane, channel, block, half)
+    uint8_t *d0 = p0 + (row * Np + e_base) * 16;
+    uint8_t *d1 = p1 + (row * Np + e_base) * 8;
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-      uint8_t *d0 = p0 + ((blk + h) * Np + e_base) * 16;
-      uint8_t *d1 = p1 + ((blk + h) * Np + e_base) * 8;
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const int el = k * 256 + tid;
-        if (el < n_el) {
-          const int slot = el + (el >> 4);
-          *(v4u *)(d0 + el * 16) = s0[h][slot];
-          *(v2u *)(d1 + el * 8) = s1[h][slot];
-        }
+    for (int k = 0; k < 4; k++) {
+      const int el = k * 256 + tid;
+      if (el < n_el) {
+        const int slot = el + (el >> 4);
+        *(v4u *)(d0 + el * 16) = s0[slot];
+        *(v2u *)(d1 + el * 8) = s1[slot];
       }
     }
     __syncthreads();
@@ -306,10 +301,13 @@ __device__ __forceinline__ void gemm_tile_f6(const Fp6Job &J, const int f, const
   Ops o[2];
   read_ops(o[0], 0);
 
-  // Stage st (slot st % kRing): refill the slot of stage st-1 with stage
-  // st+kRing-1; wait for this wave's loads of st+1, barrier (all waves' loads
-  // landed, all reads of the refilled slot done); read st+1's operands while
-  // st's MFMAs run.
+  // Stage st (slot st % kRing):
+  //   wait for this wave's loads of st+1 (stages st+2..st+kRing-2 may stay
+  //   pending: 3 * 6 loads), barrier: every wave's loads of st+1 landed and
+  //   every wave's reads of the slot of st-1 are done;
+  //   read st+1's operands, two LDS reads per MFMA over st's first 8 MFMAs;
+  //   refill the slot of st-1 with stage st+kRing-1, one DMA per MFMA over
+  //   the remaining ones.
   for (int c0 = 0; c0 < nst; c0 += kChunkStages) {
     const int c1 = c0 + kChunkStages < nst ? c0 + kChunkStages : nst;
     for (int st0 = c0; st0 < c1; st0 += kRing) {
@@ -317,10 +315,24 @@ __device__ __forceinline__ void gemm_tile_f6(const Fp6Job &J, const int f, const
       for (int d = 0; d < kRing; d++) {
         const int st = st0 + d;
         const int nx = st + kRing - 1;
-        issue_stage(nx < nst ? nx : nst - 1, (d + kRing - 1) % kRing);
-        asm volatile("s_waitcnt vmcnt(24)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+        asm volatile("s_waitcnt vmcnt(18)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
         read_ops(o[(d + 1) & 1], (d + 1) % kRing);
         mfma_ops(o[d & 1]);
+        issue_stage(nx < nst ? nx : nst - 1, (d + kRing - 1) % kRing);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+          __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // LDS read
+        }
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+          __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM (LDS DMA)
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
 #pragma unroll
@@ -364,6 +376,7 @@ static int64_t f6_chunk_samples(const chx_plan *p) {
   int64_t tc = kWorkspaceBytes / per_sample;
   const int64_t desc = 0x7FFFFFFFLL / (Np / 2);  // KB * 2 * Np * 16 = tc * Np / 2 bytes
   if (tc > desc) tc = desc;
+  if (tc > (int64_t)kK * 32767) tc = (int64_t)kK * 32767;  // unpack grid.y = 2 KB <= 65535
   tc = tc / unit * unit;
   const int64_t T = (p->cfg.time_steps + unit - 1) / unit * unit;
   if (tc > T) tc = T;
@@ -397,7 +410,7 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
       p->prof_ev->insert(p->prof_ev->end(), ev, ev + 3);
       CHX_HIP(hipEventRecord(ev[0], s));
     }
-    unpack_f6_kernel<<<dim3((unsigned)((Np + 1023) / 1024), (unsigned)KB, (unsigned)F),
+    unpack_f6_kernel<<<dim3((unsigned)((Np + 1023) / 1024), (unsigned)(2 * KB), (unsigned)F),
                        dim3(256), 0, s>>>(in, p0, p1, N, F, Np, row_bytes, t0, tv, KB);
     CHX_HIP(hipGetLastError());
     if (p->prof) CHX_HIP(hipEventRecord(ev[1], s));

@@ -401,6 +401,7 @@ static int64_t chunk_samples(const chx_plan *p) {
   int64_t tc = kWorkspaceBytes / per_sample;
   // one (plane, channel) region must stay addressable by a buffer descriptor
   if (tc > 0x7FFFFFFFLL / Np) tc = 0x7FFFFFFFLL / Np;
+  if (tc > 16LL * 65535) tc = 16LL * 65535;  // unpack grid.y = KB <= 65535
   tc = tc / (kStage * kRing) * (kStage * kRing);
   const int64_t T = (p->cfg.time_steps + kStage * kRing - 1) / (kStage * kRing) * (kStage * kRing);
   if (tc > T) tc = T;

@@ -184,3 +184,12 @@ def test_methods_agree_on_array_scale_channel_slice():
                        (chx.HIGHLY_PACKED, 512)):
         np.testing.assert_array_equal(run(data, method, 1, ta), want,
                                       err_msg=chx.METHOD_NAMES[method])
+
+
+@pytest.mark.parametrize("method", MATRIX)
+def test_matrix_long_small_array_multi_chunk(method):
+    # A small array over 2^21 + 96 samples: several plane chunks (each capped
+    # so the unpack grid stays within its limits) and a ragged last chunk.
+    T, F, N = (1 << 21) + 96, 1, 32
+    data = rand(T, F, N, 21)
+    np.testing.assert_array_equal(run(data, method, 1, 32), O.correlate_blocks(data, 1))
