@@ -1,6 +1,7 @@
 """Autotuner of the batch launch shape (chx_batch_tuning).
 
-For a job list it times every candidate shape -- the work-item time extent
+For a job list it times every candidate shape -- the matrix-core method
+(FP6 or int8 operands; bit-identical results), the work-item time extent
 (max_item_stages: whole chunks, or pieces of 64..512 stages whose partial
 sums add atomically), the persistent workgroup count, and the workspace
 budget (which sets the number of rounds) -- on the GPU with the captured
@@ -37,11 +38,12 @@ def signature(jobs) -> str:
 
 
 def candidates(cus: int):
-    stages = (0, 512, 256, 128, 64)
+    methods = (3, 2)  # FP6, int8 matrix cores (bit-identical results)
+    stages = (0, 510, 252, 126, 66)  # whole chunk, or pieces (rounded to the ring)
     wgs = (cus,)
     budgets = (16 << 30, 64 << 30)
-    for st, wg, ws in itertools.product(stages, wgs, budgets):
-        yield dict(max_item_stages=st, workgroups=wg, workspace_bytes=ws)
+    for m, st, wg, ws in itertools.product(methods, stages, wgs, budgets):
+        yield dict(method=m, max_item_stages=st, workgroups=wg, workspace_bytes=ws)
 
 
 def time_batch(descs, tuning, reps: int = 3) -> float:

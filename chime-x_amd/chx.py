@@ -77,6 +77,8 @@ class BatchTuning(ctypes.Structure):
         ("max_item_stages", ctypes.c_int32),
         ("workgroups", ctypes.c_int32),
         ("workspace_bytes", ctypes.c_int64),
+        ("method", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -90,6 +92,8 @@ class BatchInfo(ctypes.Structure):
         ("max_item_stages", ctypes.c_int32),
         ("workspace_bytes", ctypes.c_size_t),
         ("cmacs", ctypes.c_int64),
+        ("method", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -105,6 +109,8 @@ class ServeConfig(ctypes.Structure):
         ("hbm_budget_bytes", ctypes.c_int64),
         ("max_batch_frames", ctypes.c_int32),
         ("max_item_stages", ctypes.c_int32),
+        ("method", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -321,7 +327,7 @@ def _tuning(tuning):
     if not tuning:
         return None
     return BatchTuning(tuning.get("max_item_stages", 0), tuning.get("workgroups", 0),
-                       tuning.get("workspace_bytes", 0))
+                       tuning.get("workspace_bytes", 0), tuning.get("method", 0), 0)
 
 
 def batch_plan(jobs, tuning: dict | None = None) -> dict:
@@ -341,7 +347,7 @@ class Batch:
     jobs: sequence of dicts with num_elements, num_freq, time_steps,
     upper_triangle_convention, d_input, d_vis (device pointers as ints).
     tuning: None (defaults) or a dict with any of max_item_stages,
-    workgroups, workspace_bytes.
+    workgroups, workspace_bytes, method (INT8_MFMA or FP6_MFMA; default FP6).
     """
 
     def __init__(self, jobs, tuning: dict | None = None, device: int = 0):
@@ -393,9 +399,9 @@ class Serve:
 
     def __init__(self, frame_slots: int, slot_bytes: int, workspace_bytes: int = 0,
                  hbm_budget_bytes: int = 0, max_batch_frames: int = 0,
-                 max_item_stages: int = 0, device: int = 0):
+                 max_item_stages: int = 0, device: int = 0, method: int = 0):
         self.cfg = ServeConfig(device, frame_slots, slot_bytes, workspace_bytes,
-                               hbm_budget_bytes, max_batch_frames, max_item_stages)
+                               hbm_budget_bytes, max_batch_frames, max_item_stages, method, 0)
         h = ctypes.c_void_p()
         _check(lib().chx_serve_create(ctypes.byref(self.cfg), ctypes.byref(h)),
                "chx_serve_create")

@@ -24,6 +24,7 @@
 
 struct chx_batch {
   int device = 0;
+  int fmt = 1;  // chx::PlaneFormat
   std::vector<chx_job_desc> jobs;
   chx_batch_tuning tune{};
   int64_t chunk = 0;
@@ -93,29 +94,38 @@ void spread_over_queues(std::vector<ItemGroup> &groups, std::vector<GemmItem> &i
   qbound.push_back((int)items.size());
 }
 
-int check_launch_tables(const std::vector<UnpackJob> &ujobs, const std::vector<GemmJob> &gjobs,
-                        const std::vector<GemmItem> &items, const std::vector<int> &qbound,
-                        int rounds, const uint8_t *base, size_t work_bytes) {
+int check_launch_tables(int fmt, const std::vector<UnpackJob> &ujobs,
+                        const std::vector<GemmJob> &gjobs, const std::vector<GemmItem> &items,
+                        const std::vector<int> &qbound, int rounds, const uint8_t *base,
+                        size_t work_bytes) {
   const uint8_t *end = base + work_bytes;
+  const int ring = format_ring(fmt);
+  const int spb = fmt == kPlanesFp6 ? 64 : 16;  // samples per layout block
   for (const UnpackJob &u : ujobs) {
-    const int64_t bytes = 2LL * u.F * u.KB_dst * u.Np * 16;
-    if (!u.in || u.planes < base || u.planes + bytes > end || u.t_valid < 1 ||
-        u.t_valid > 16LL * u.KB || u.Np % kMfmaTile || u.N > u.Np || u.kb_dst0 < 0 ||
+    const int64_t stages = fmt == kPlanesFp6 ? u.KB_dst : u.KB_dst / 4;
+    const int64_t bytes = format_plane_bytes(fmt, u.F, u.Np, stages);
+    if (!u.in || u.planes < base || u.planes + bytes > end || u.t_valid < 0 ||
+        u.t_valid > (int64_t)spb * u.KB || u.Np % kMfmaTile || u.N > u.Np || u.kb_dst0 < 0 ||
         u.kb_dst0 + u.KB > u.KB_dst)
+      return CHX_ERR_HIP;
+    if (fmt == kPlanesFp6 && u.planes1 != u.planes + 64LL * u.F * u.KB_dst * u.Np)
       return CHX_ERR_HIP;
   }
   for (const GemmJob &g : gjobs) {
-    const int64_t bytes = 2LL * g.F * g.KB * g.Np * 16;
-    if (!g.vis || g.planes < base || g.planes + bytes > end || g.KB % (4 * kMfmaRing) ||
-        g.ntiles * kMfmaTile != g.Np || (int64_t)g.KB * g.Np * 16 > 0x7FFFFFFFLL)
+    const int stages = fmt == kPlanesFp6 ? g.KB : g.KB / 4;
+    const int64_t bytes = format_plane_bytes(fmt, g.F, g.Np, stages);
+    const int64_t region = fmt == kPlanesFp6 ? 32LL * g.KB * g.Np : 16LL * g.KB * g.Np;
+    if (!g.vis || g.planes < base || g.planes + bytes > end || stages % ring ||
+        format_kb(fmt, stages) != g.KB || g.ntiles * kMfmaTile != g.Np || region > 0x7FFFFFFFLL)
       return CHX_ERR_HIP;
+    if (fmt == kPlanesFp6 && g.planes1 != g.planes + 64LL * g.F * g.KB * g.Np) return CHX_ERR_HIP;
   }
   for (const GemmItem &w : items) {
     if (w.job < 0 || w.job >= (int)gjobs.size()) return CHX_ERR_HIP;
     const GemmJob &g = gjobs[w.job];
+    const int stages = fmt == kPlanesFp6 ? g.KB : g.KB / 4;
     if (w.f < 0 || w.f >= g.F || w.tile < 0 || w.tile >= g.ntiles * (g.ntiles + 1) / 2 ||
-        w.stage0 < 0 || w.nstages < kMfmaRing || w.nstages % kMfmaRing ||
-        w.stage0 + w.nstages > g.KB / 4)
+        w.stage0 < 0 || w.nstages < ring || w.nstages % ring || w.stage0 + w.nstages > stages)
       return CHX_ERR_HIP;
   }
   if ((int)qbound.size() != 9 * rounds || qbound.empty() ||
@@ -127,9 +137,21 @@ int check_launch_tables(const std::vector<UnpackJob> &ujobs, const std::vector<G
   return CHX_SUCCESS;
 }
 
+int launch_unpack_batch(int fmt, const UnpackJob *d_jobs, int njobs, int64_t units,
+                        hipStream_t s) {
+  return fmt == kPlanesFp6 ? launch_unpack_batch_fp6(d_jobs, njobs, units, s)
+                           : launch_unpack_batch_int8(d_jobs, njobs, units, s);
+}
+
+int launch_gemm_batch(int fmt, const GemmJob *d_jobs, const GemmItem *d_items,
+                      const int *d_qbound, int *d_counters, int nwg, hipStream_t s) {
+  return fmt == kPlanesFp6
+             ? launch_gemm_batch_fp6(d_jobs, d_items, d_qbound, d_counters, nwg, s)
+             : launch_gemm_batch_int8(d_jobs, d_items, d_qbound, d_counters, nwg, s);
+}
+
 namespace {
 
-constexpr int64_t kRoundSamples = (int64_t)kMfmaStage * kMfmaRing;  // 256
 constexpr int64_t kDefaultBudget = 16LL << 30;
 
 int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
@@ -183,29 +205,34 @@ int plan_sizes(chx_batch *b, std::vector<int64_t> &off) {
   std::vector<int64_t> Np(nj);
   off.assign(nj, 0);
   b->cmacs = 0;
-  int64_t per_sample = 0, maxT = 0;
+  const int64_t unit = (int64_t)kMfmaStage * format_ring(b->fmt);  // samples per ring
+  int64_t per_unit = 0, maxT = 0;  // plane bytes of one ring of all jobs
   for (int j = 0; j < nj; j++) {
     const chx_job_desc &d = b->jobs[j];
     Np[j] = round_up(d.num_elements, kMfmaTile);
-    per_sample += 2 * (int64_t)d.num_freq * Np[j];
+    per_unit += format_plane_bytes(b->fmt, d.num_freq, Np[j], format_ring(b->fmt));
     maxT = std::max(maxT, d.time_steps);
     const int64_t N = d.num_elements;
     b->cmacs += (int64_t)d.num_freq * d.time_steps * (N * (N + 1) / 2);
   }
-  // Chunk: as long as the budget allows (a multiple of 256 samples), no
-  // longer than the longest job, and short enough that one (plane, channel)
-  // region stays addressable by a 32-bit buffer descriptor.
-  int64_t chunk = b->tune.workspace_bytes / per_sample / kRoundSamples * kRoundSamples;
-  if (chunk < kRoundSamples) return CHX_ERR_BUDGET;
-  chunk = std::min(chunk, round_up(maxT, kRoundSamples));
-  for (int j = 0; j < nj; j++)
-    chunk = std::min<int64_t>(chunk, 0x7FFFFFFFLL / Np[j] / kRoundSamples * kRoundSamples);
+  // Chunk: as long as the budget allows (whole rings), no longer than the
+  // longest job, and short enough that one (plane, channel) region stays
+  // addressable by a 32-bit buffer descriptor (int8: 2 bytes per element and
+  // sample, i.e. chunk * Np; fp6: chunk * Np / 2 per array).
+  int64_t chunk = b->tune.workspace_bytes / per_unit * unit;
+  if (chunk < unit) return CHX_ERR_BUDGET;
+  chunk = std::min(chunk, round_up(maxT, unit));
+  for (int j = 0; j < nj; j++) {
+    const int64_t lim = b->fmt == kPlanesFp6 ? 0x7FFFFFFFLL / (Np[j] / 2) : 0x7FFFFFFFLL / Np[j];
+    chunk = std::min<int64_t>(chunk, lim / unit * unit);
+  }
   b->chunk = chunk;
   b->rounds = (int)((maxT + chunk - 1) / chunk);
   int64_t ws = 0;
   for (int j = 0; j < nj; j++) {
     off[j] = ws;
-    ws += 2 * (int64_t)b->jobs[j].num_freq * Np[j] * std::min(chunk, round_up(b->jobs[j].time_steps, kRoundSamples));
+    const int64_t stages = std::min(chunk, round_up(b->jobs[j].time_steps, unit)) / kMfmaStage;
+    ws += format_plane_bytes(b->fmt, b->jobs[j].num_freq, Np[j], stages);
   }
   b->work_bytes = (size_t)ws;
   return CHX_SUCCESS;
@@ -222,7 +249,8 @@ int plan_tables(chx_batch *b, uint8_t *base, const std::vector<int64_t> &off, Ta
   std::vector<int> &qbound = t.qbound;
   b->round.clear();
   b->zero.clear();
-  const int max_st = b->tune.max_item_stages;  // multiple of kMfmaRing, 0 = none
+  const int max_st = b->tune.max_item_stages;  // multiple of the ring, 0 = none
+  const int64_t unit = (int64_t)kMfmaStage * format_ring(b->fmt);
   for (int r = 0; r < b->rounds; r++) {
     chx_batch::Round R;
     R.ujob0 = (int)ujobs.size();
@@ -232,16 +260,18 @@ int plan_tables(chx_batch *b, uint8_t *base, const std::vector<int64_t> &off, Ta
       const int64_t t0 = (int64_t)r * chunk;
       if (t0 >= d.time_steps) continue;
       const int64_t tv = std::min(chunk, d.time_steps - t0);
-      const int nst = (int)(round_up(tv, kRoundSamples) / kMfmaStage);
-      const int KB = nst * (kMfmaStage / 16);
+      const int nst = (int)(round_up(tv, unit) / kMfmaStage);
+      const int KB = format_kb(b->fmt, nst);
       const int F = d.num_freq, N = d.num_elements;
       const int nt = (N + kMfmaTile - 1) / kMfmaTile;
       const int64_t Npj = round_up(N, kMfmaTile);
       uint8_t *planes = base + off[j];
+      uint8_t *planes1 = b->fmt == kPlanesFp6 ? planes + 64LL * F * KB * Npj : nullptr;
 
       UnpackJob u;
       u.in = d.d_input;
       u.planes = planes;
+      u.planes1 = planes1;
       u.row_bytes = (int64_t)F * N;
       u.t0 = t0;
       u.t_valid = tv;
@@ -253,13 +283,14 @@ int plan_tables(chx_batch *b, uint8_t *base, const std::vector<int64_t> &off, Ta
       u.KB_dst = KB;
       u.kb_dst0 = 0;
       ujobs.push_back(u);
-      R.units += (int64_t)((Npj + 1023) / 1024) * KB * F;
+      R.units += unpack_units(b->fmt, u);
 
       // Items of this job: whole chunk per tile, or max_st-stage pieces.
       const int per = (max_st > 0 && max_st < nst) ? max_st : nst;
       const int pieces = add_job_items((int)gjobs.size(), F, nt, nst, max_st, groups);
       GemmJob g;
       g.planes = planes;
+      g.planes1 = planes1;
       g.vis = d.d_vis;
       g.NB = chx_num_blocks(N);
       g.F = F;
@@ -283,7 +314,7 @@ int plan_tables(chx_batch *b, uint8_t *base, const std::vector<int64_t> &off, Ta
 }
 
 int check_tables(const chx_batch *b, const uint8_t *base, const Tables &t) {
-  return check_launch_tables(t.ujobs, t.gjobs, t.items, t.qbound, b->rounds, base,
+  return check_launch_tables(b->fmt, t.ujobs, t.gjobs, t.items, t.qbound, b->rounds, base,
                              b->work_bytes);
 }
 
@@ -313,9 +344,9 @@ int enqueue(const chx_batch *b, hipStream_t s) {
     if (!rc) rc = launch_fill_u32((uint32_t *)z.p, 0u, z.n, s);
   for (int r = 0; r < b->rounds && !rc; r++) {
     const chx_batch::Round &R = b->round[r];
-    rc = launch_unpack_batch(b->d_ujobs + R.ujob0, R.nujobs, R.units, s);
+    rc = launch_unpack_batch(b->fmt, b->d_ujobs + R.ujob0, R.nujobs, R.units, s);
     if (!rc)
-      rc = launch_gemm_batch(b->d_gjobs, b->d_items, b->d_qbound + 9 * r,
+      rc = launch_gemm_batch(b->fmt, b->d_gjobs, b->d_items, b->d_qbound + 9 * r,
                              b->d_counters + 8 * r, b->tune.workgroups, s);
   }
   return rc;
@@ -323,7 +354,7 @@ int enqueue(const chx_batch *b, hipStream_t s) {
 
 // Validation shared by chx_batch_create and chx_batch_plan (host only).
 int prepare(const chx_job_desc *jobs, int32_t num_jobs, const chx_batch_tuning *tuning,
-            chx_batch_tuning &t) {
+            chx_batch_tuning &t, int &fmt) {
   if (!jobs || num_jobs < 1) return CHX_ERR_INVALID_ARGUMENT;
   for (int j = 0; j < num_jobs; j++) {
     int rc = validate_job(jobs[j]);
@@ -333,13 +364,18 @@ int prepare(const chx_job_desc *jobs, int32_t num_jobs, const chx_batch_tuning *
   if (t.max_item_stages < 0 || t.workgroups < 0 || t.workgroups % 8 != 0 ||
       t.workspace_bytes < 0)
     return CHX_ERR_INVALID_ARGUMENT;
+  if (t.method == 0) t.method = CHX_METHOD_FP6_MFMA;
+  if (t.method != CHX_METHOD_INT8_MFMA && t.method != CHX_METHOD_FP6_MFMA)
+    return CHX_ERR_INVALID_ARGUMENT;
+  fmt = t.method == CHX_METHOD_FP6_MFMA ? kPlanesFp6 : kPlanesInt8;
   if (t.workspace_bytes == 0) t.workspace_bytes = kDefaultBudget;
-  t.max_item_stages = (int32_t)round_up(t.max_item_stages, kMfmaRing);
-  // The smallest round (256 samples of every job) must fit the budget.
-  int64_t per_sample = 0;
+  t.max_item_stages = (int32_t)round_up(t.max_item_stages, format_ring(fmt));
+  // The smallest round (one ring of every job) must fit the budget.
+  int64_t per_unit = 0;
   for (int j = 0; j < num_jobs; j++)
-    per_sample += 2 * (int64_t)jobs[j].num_freq * round_up(jobs[j].num_elements, kMfmaTile);
-  if (per_sample > t.workspace_bytes / kRoundSamples) return CHX_ERR_BUDGET;
+    per_unit += format_plane_bytes(fmt, jobs[j].num_freq,
+                                   round_up(jobs[j].num_elements, kMfmaTile), format_ring(fmt));
+  if (per_unit > t.workspace_bytes) return CHX_ERR_BUDGET;
   return CHX_SUCCESS;
 }
 
@@ -355,7 +391,8 @@ int chx_batch_create(int32_t device, const chx_job_desc *jobs, int32_t num_jobs,
   if (!out) return CHX_ERR_INVALID_ARGUMENT;
   *out = nullptr;
   chx_batch_tuning t;
-  int rc = prepare(jobs, num_jobs, tuning, t);
+  int fmt = kPlanesFp6;
+  int rc = prepare(jobs, num_jobs, tuning, t, fmt);
   if (rc) return rc;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return CHX_ERR_NO_DEVICE;
@@ -369,6 +406,7 @@ int chx_batch_create(int32_t device, const chx_job_desc *jobs, int32_t num_jobs,
   chx_batch *b = new (std::nothrow) chx_batch;
   if (!b) return CHX_ERR_OUT_OF_MEMORY;
   b->device = device;
+  b->fmt = fmt;
   b->jobs.assign(jobs, jobs + num_jobs);
   b->tune = t;
   rc = plan_batch(b);
@@ -384,9 +422,11 @@ int chx_batch_plan(const chx_job_desc *jobs, int32_t num_jobs, const chx_batch_t
                    chx_batch_info *info) {
   if (!info) return CHX_ERR_INVALID_ARGUMENT;
   chx_batch_tuning t;
-  int rc = prepare(jobs, num_jobs, tuning, t);
+  int fmt = kPlanesFp6;
+  int rc = prepare(jobs, num_jobs, tuning, t, fmt);
   if (rc) return rc;
   chx_batch b;
+  b.fmt = fmt;
   b.jobs.assign(jobs, jobs + num_jobs);
   b.tune = t;
   std::vector<int64_t> off;
@@ -439,6 +479,7 @@ int chx_batch_get_info(const chx_batch *b, chx_batch_info *info) {
   info->items = b->nitems;
   info->workgroups = b->tune.workgroups;
   info->max_item_stages = b->tune.max_item_stages;
+  info->method = b->tune.method;
   info->workspace_bytes = b->work_bytes;
   info->cmacs = b->cmacs;
   return CHX_SUCCESS;

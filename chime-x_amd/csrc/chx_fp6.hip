@@ -92,16 +92,19 @@ __device__ __forceinline__ void transpose4(uint32_t r0, uint32_t r1, uint32_t r2
 // sample row; 256 contiguous bytes per wave and row), 8 byte transposes,
 // then per plane: codes, bit packing, LDS staging, and full-run global
 // stores (1 KB of P0 or 512 B of P1 per wave instruction).
-__global__ __launch_bounds__(256) void unpack_f6_kernel(
-    const uint8_t *__restrict__ in, uint8_t *__restrict__ p0, uint8_t *__restrict__ p1, int N,
-    int F, int Np, int64_t row_bytes, int64_t t0, int64_t t_valid, int KB) {
-  __shared__ __attribute__((aligned(16))) v4u s0[1024 + 64];  // [element] P0, padded
-  __shared__ __attribute__((aligned(16))) v2u s1[1024 + 64];  // [element] P1, padded
+// KB is the destination layout's block count; (kb, h) the destination
+// block and half written; source sample of its first row = 64 (kb -
+// kb_src0) + 32 h.
+__device__ __forceinline__ void unpack_f6_unit(const uint8_t *__restrict__ in,
+                                               uint8_t *__restrict__ p0,
+                                               uint8_t *__restrict__ p1, int N, int F, int Np,
+                                               int64_t row_bytes, int64_t t0, int64_t t_valid,
+                                               int KB, int qb, int kb, int kb_src0, int h, int f,
+                                               v4u *s0, v2u *s1) {
   const int tid = threadIdx.x;
-  const int q = blockIdx.x * 256 + tid;  // element quad
-  const int kb = blockIdx.y >> 1, h = blockIdx.y & 1, f = blockIdx.z;
+  const int q = qb * 256 + tid;  // element quad
   const bool evalid = 4 * q < N;
-  const int64_t ts = (int64_t)kK * kb + 32 * h;  // first sample of the half
+  const int64_t ts = (int64_t)kK * (kb - kb_src0) + 32 * h;  // first source sample of the half
   uint32_t c[4][8];  // [element][4-sample group]
   {
     const uint8_t *src = in + (t0 + ts) * row_bytes + (int64_t)f * N + 4 * q;
@@ -116,7 +119,7 @@ __global__ __launch_bounds__(256) void unpack_f6_kernel(
       transpose4(w[4 * m], w[4 * m + 1], w[4 * m + 2], w[4 * m + 3], c[0][m], c[1][m], c[2][m],
                  c[3][m]);
   }
-  const int e_base = blockIdx.x * 1024;
+  const int e_base = qb * 1024;
   const int n_el = Np - e_base < 1024 ? Np - e_base : 1024;
 #pragma unroll
   for (int pl = 0; pl < 2; pl++) {
@@ -149,6 +152,44 @@ __global__ __launch_bounds__(256) void unpack_f6_kernel(
     }
     __syncthreads();
   }
+}
+
+// Pass 1.  Workgroup = 256 element quads (1024 elements) x one 32-sample
+// half of a 64-sample block of one channel.  Thread: 32 word loads (one per
+// sample row; 256 contiguous bytes per wave and row), 8 byte transposes,
+// then per plane: codes, bit packing, LDS staging, and full-run global
+// stores (1 KB of P0 or 512 B of P1 per wave instruction).
+__global__ __launch_bounds__(256) void unpack_f6_kernel(
+    const uint8_t *__restrict__ in, uint8_t *__restrict__ p0, uint8_t *__restrict__ p1, int N,
+    int F, int Np, int64_t row_bytes, int64_t t0, int64_t t_valid, int KB) {
+  __shared__ __attribute__((aligned(16))) v4u s0[1024 + 64];  // [element] P0, padded
+  __shared__ __attribute__((aligned(16))) v2u s1[1024 + 64];  // [element] P1, padded
+  unpack_f6_unit(in, p0, p1, N, F, Np, row_bytes, t0, t_valid, KB, blockIdx.x, blockIdx.y >> 1,
+                 0, blockIdx.y & 1, blockIdx.z, s0, s1);
+}
+
+// Batch: one launch unpacks the chunks of many jobs; workgroup g belongs to
+// the job j with jobs[j].unit0 <= g < jobs[j+1].unit0 (binary search); its
+// unit within the job is (channel, block, half, element group).
+__global__ __launch_bounds__(256) void unpack_f6_batch_kernel(const UnpackJob *__restrict__ jobs,
+                                                              int njobs) {
+  __shared__ __attribute__((aligned(16))) v4u s0[1024 + 64];
+  __shared__ __attribute__((aligned(16))) v2u s1[1024 + 64];
+  const int64_t g = blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].unit0 <= g) lo = mid; else hi = mid - 1;
+  }
+  const UnpackJob &J = jobs[lo];
+  const int64_t u = g - J.unit0;
+  const int nq = (J.Np + 1023) / 1024;
+  if (u >= (int64_t)nq * 2 * J.KB * J.F) return;  // grid padding (uniform per workgroup)
+  const int qb = (int)(u % nq);
+  const int hk = (int)((u / nq) % (2 * J.KB));  // (block, half) within the job
+  const int f = (int)(u / ((int64_t)nq * 2 * J.KB));
+  unpack_f6_unit(J.in, J.planes, J.planes1, J.N, J.F, J.Np, J.row_bytes, J.t0, J.t_valid,
+                 J.KB_dst, qb, J.kb_dst0 + (hk >> 1), J.kb_dst0, hk & 1, f, s0, s1);
 }
 
 // Flush one block's FP32 accumulators (exact integers) into int32
@@ -198,7 +239,7 @@ __device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti, int &tj
 
 // Pass 2, one output tile (ti, tj) of channel f over k-steps
 // [st_begin, st_begin + nst) (nst a multiple of kRing).
-__device__ __forceinline__ void gemm_tile_f6(const Fp6Job &J, const int f, const int ti,
+__device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, const int ti,
                                              const int tj, const int st_begin, const int nst,
                                              uint8_t (*lds)[kSlotBytes]) {
   const int F = J.F, Np = J.Np, KB = J.KB, nb = J.nb, U = J.U, mode = J.mode;
@@ -215,10 +256,10 @@ __device__ __forceinline__ void gemm_tile_f6(const Fp6Job &J, const int f, const
   const int64_t p0_blk = (int64_t)2 * Np * 16;  // one 64-sample block of one channel
   const int64_t p1_blk = (int64_t)2 * Np * 8;
   const __amdgpu_buffer_rsrc_t r0 = __builtin_amdgcn_make_buffer_rsrc(
-      (void *)(J.p0 + ((int64_t)lp * F + f) * KB * p0_blk), (short)0, (int)(KB * p0_blk),
+      (void *)(J.planes + ((int64_t)lp * F + f) * KB * p0_blk), (short)0, (int)(KB * p0_blk),
       0x00020000);
   const __amdgpu_buffer_rsrc_t r1 = __builtin_amdgcn_make_buffer_rsrc(
-      (void *)(J.p1 + ((int64_t)lp * F + f) * KB * p1_blk), (short)0, (int)(KB * p1_blk),
+      (void *)(J.planes1 + ((int64_t)lp * F + f) * KB * p1_blk), (short)0, (int)(KB * p1_blk),
       0x00020000);
   int voff0[4], voff1[2];
 #pragma unroll
@@ -354,7 +395,7 @@ __device__ __forceinline__ void gemm_tile_f6(const Fp6Job &J, const int f, const
 
 // One plan: grid.x = channels x upper-triangle tiles (XCD-remapped), grid.y
 // = time splits.
-__global__ __launch_bounds__(256, 1) void corr_f6_kernel(Fp6Job J) {
+__global__ __launch_bounds__(256, 1) void corr_f6_kernel(GemmJob J) {
   __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kSlotBytes];
   const int64_t item = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
   const int64_t per_f = (int64_t)J.ntiles * (J.ntiles + 1) / 2;
@@ -364,7 +405,59 @@ __global__ __launch_bounds__(256, 1) void corr_f6_kernel(Fp6Job J) {
                J.stages_per_split, lds);
 }
 
+// Batch: persistent workgroups take work items from 8 per-XCD queues (the
+// same scheme as the int8 batch kernel: drain the own queue, then move on;
+// queues never refill, so every workgroup ends).
+__global__ __launch_bounds__(256, 1) void corr_f6_batch_kernel(
+    const GemmJob *__restrict__ jobs, const GemmItem *__restrict__ items,
+    const int *__restrict__ qbound, int *__restrict__ counters) {
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kSlotBytes];
+  __shared__ int next;
+  int q = blockIdx.x % 8, moves = 0;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int it = -1;
+      while (moves < 8) {
+        const int k = atomicAdd(&counters[q], 1) + qbound[q];
+        if (k < qbound[q + 1]) {
+          it = k;
+          break;
+        }
+        q = (q + 1) % 8;
+        moves++;
+      }
+      next = it;
+    }
+    __syncthreads();
+    const int it = next;
+    __syncthreads();
+    if (it < 0) break;
+    const GemmItem w = items[it];
+    const GemmJob J = jobs[w.job];
+    int ti, tj;
+    tile_coords(w.tile, J.ntiles, ti, tj);
+    gemm_tile_f6(J, w.f, ti, tj, w.stage0, w.nstages, lds);
+  }
+}
+
 }  // namespace
+
+int launch_unpack_batch_fp6(const UnpackJob *d_jobs, int njobs, int64_t units, hipStream_t s) {
+  if (units <= 0) return CHX_SUCCESS;
+  const int64_t gx = units < 65536 ? units : 65536;
+  const int64_t gy = (units + gx - 1) / gx;
+  unpack_f6_batch_kernel<<<dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s>>>(d_jobs, njobs);
+  CHX_HIP(hipGetLastError());
+  return CHX_SUCCESS;
+}
+
+int launch_gemm_batch_fp6(const GemmJob *d_jobs, const GemmItem *d_items, const int *d_qbound,
+                          int *d_counters, int nwg, hipStream_t s) {
+  corr_f6_batch_kernel<<<dim3((unsigned)nwg), dim3(256), 0, s>>>(d_jobs, d_items, d_qbound,
+                                                                 d_counters);
+  CHX_HIP(hipGetLastError());
+  return CHX_SUCCESS;
+}
 
 // Samples per chunk: planes of one chunk within kWorkspaceBytes, a
 // multiple of the ring (kRing k-steps), one (array, plane, channel) region
@@ -428,9 +521,9 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
       }
       kmode = 2;
     }
-    Fp6Job J;
-    J.p0 = p0;
-    J.p1 = p1;
+    GemmJob J;
+    J.planes = p0;
+    J.planes1 = p1;
     J.vis = vis;
     J.NB = p->NB;
     J.F = F;

@@ -67,23 +67,33 @@ constexpr int kFp6Ring = 6;  // 64-sample stages in the LDS ring
 size_t fp6_workspace_bytes(const chx_plan *p);
 int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode,
                     hipStream_t s);
-struct Fp6Job {
-  const uint8_t *p0, *p1;  // E3M2 planes of the chunk: bytes 0-15 / 16-23 of each half
-  int32_t *vis;
-  int64_t NB;
-  int F, Np, KB, ntiles, nb, U, mode, stages_per_split;  // KB: 64-sample blocks
-};
+
+// Plane formats of the matrix-core GEMMs.  A stage is 64 samples in both;
+// stage counts of a job are multiples of the format's ring.
+//  int8: planes [2][F][KB][Np][16], KB = 16-sample blocks, 2 bytes per
+//        (element, sample).
+//  fp6:  P0 [2][F][KB][2 halves][Np][16] then P1 [2][F][KB][2][Np][8], KB =
+//        64-sample blocks, 1.5 bytes per (element, sample).
+enum PlaneFormat { kPlanesInt8 = 0, kPlanesFp6 = 1 };
+inline int format_ring(int fmt) { return fmt == kPlanesFp6 ? kFp6Ring : kMfmaRing; }
+// plane bytes of one job chunk of `stages` 64-sample stages (both planes)
+inline int64_t format_plane_bytes(int fmt, int64_t F, int64_t Np, int64_t stages) {
+  return fmt == kPlanesFp6 ? 96 * F * Np * stages : 128 * F * Np * stages;
+}
+// KB (the layout's block count) of `stages` stages
+inline int format_kb(int fmt, int stages) { return fmt == kPlanesFp6 ? stages : 4 * stages; }
 
 // One job as the matrix-core kernels see it (per time chunk).
 struct GemmJob {
-  const uint8_t *planes;  // int8 planes [2][F][KB][Np][16] of the chunk
-  int32_t *vis;           // block-order output
-  int64_t NB;             // blocks per channel
+  const uint8_t *planes;   // int8 planes, or fp6 P0
+  const uint8_t *planes1;  // fp6 P1 (unused for int8)
+  int32_t *vis;            // block-order output
+  int64_t NB;              // blocks per channel
   int F, Np, KB, ntiles, nb, U, mode, stages_per_split;  // mode 0 =, 1 +=, 2 atomic +=
 };
 // One work item of the persistent batch kernel: tile `tile` (row-major over
 // the upper triangle of tiles) of channel f of jobs[job], over stages
-// [stage0, stage0 + nstages) of the chunk (nstages a multiple of kMfmaRing).
+// [stage0, stage0 + nstages) of the chunk (nstages a multiple of the ring).
 struct GemmItem {
   int job, f, tile, stage0, nstages;
 };
@@ -92,17 +102,23 @@ struct GemmItem {
 int launch_unpack_planes(const uint8_t *in, uint8_t *planes, int N, int F, int Np,
                          int64_t row_bytes, int64_t t0, int64_t t_valid, int KB,
                          hipStream_t s);
-// One job's chunk for the batched unpack: units = ceil(Np/1024) * KB * F
-// workgroups starting at global workgroup unit0 (increasing over jobs).
-// Its KB 16-sample blocks land at blocks [kb_dst0, kb_dst0 + KB) of planes
-// laid out as [2][F][KB_dst][Np][16].
+// One job's chunk for the batched unpack, starting at global workgroup
+// unit0 (increasing over jobs): units = ceil(Np/1024) * KB * F (int8) or
+// ceil(Np/1024) * 2 * KB * F (fp6: one workgroup per 32-sample half).  Its
+// KB blocks land at blocks [kb_dst0, kb_dst0 + KB) of planes laid out with
+// KB_dst blocks (format layouts above).  t_valid = 0 writes zeros and reads
+// nothing.
 struct UnpackJob {
   const uint8_t *in;
-  uint8_t *planes;
+  uint8_t *planes, *planes1;
   int64_t row_bytes, t0, t_valid, unit0;
   int N, F, Np, KB, KB_dst, kb_dst0;
 };
-int launch_unpack_batch(const UnpackJob *d_jobs, int njobs, int64_t units, hipStream_t s);
+inline int64_t unpack_units(int fmt, const UnpackJob &u) {
+  return (int64_t)((u.Np + 1023) / 1024) * u.KB * u.F * (fmt == kPlanesFp6 ? 2 : 1);
+}
+int launch_unpack_batch(int fmt, const UnpackJob *d_jobs, int njobs, int64_t units,
+                        hipStream_t s);
 // Work items of one (job, channel) group; groups are spread over the 8
 // per-XCD queues by spread_over_queues (chx_batch.hip).
 struct ItemGroup {
@@ -122,12 +138,20 @@ void spread_over_queues(std::vector<ItemGroup> &groups, std::vector<GemmItem> &i
 // every plane region inside the workspace [base, base + work_bytes), every
 // item inside its job's chunk (whole rings), `rounds` sets of 9 queue bounds
 // covering the items in order.  CHX_ERR_HIP if any fails.
-int check_launch_tables(const std::vector<UnpackJob> &ujobs, const std::vector<GemmJob> &gjobs,
-                        const std::vector<GemmItem> &items, const std::vector<int> &qbound,
-                        int rounds, const uint8_t *base, size_t work_bytes);
+int check_launch_tables(int fmt, const std::vector<UnpackJob> &ujobs,
+                        const std::vector<GemmJob> &gjobs, const std::vector<GemmItem> &items,
+                        const std::vector<int> &qbound, int rounds, const uint8_t *base,
+                        size_t work_bytes);
 // Persistent batch GEMM over 8 per-XCD item queues (queue q = items
 // [qbound[q], qbound[q+1])); the 8 counters must be 0.
-int launch_gemm_batch(const GemmJob *d_jobs, const GemmItem *d_items, const int *d_qbound,
-                      int *d_counters, int nwg, hipStream_t s);
+int launch_gemm_batch(int fmt, const GemmJob *d_jobs, const GemmItem *d_items,
+                      const int *d_qbound, int *d_counters, int nwg, hipStream_t s);
+// Per-format kernels behind the two launchers above.
+int launch_unpack_batch_int8(const UnpackJob *d_jobs, int njobs, int64_t units, hipStream_t s);
+int launch_gemm_batch_int8(const GemmJob *d_jobs, const GemmItem *d_items, const int *d_qbound,
+                           int *d_counters, int nwg, hipStream_t s);
+int launch_unpack_batch_fp6(const UnpackJob *d_jobs, int njobs, int64_t units, hipStream_t s);
+int launch_gemm_batch_fp6(const GemmJob *d_jobs, const GemmItem *d_items, const int *d_qbound,
+                          int *d_counters, int nwg, hipStream_t s);
 
 }  // namespace chx

@@ -453,6 +453,7 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
     }
     GemmJob J;
     J.planes = p->d_work;
+    J.planes1 = nullptr;
     J.vis = vis;
     J.NB = p->NB;
     J.F = F;
@@ -479,7 +480,7 @@ int launch_unpack_planes(const uint8_t *in, uint8_t *planes, int N, int F, int N
   return CHX_SUCCESS;
 }
 
-int launch_unpack_batch(const UnpackJob *d_jobs, int njobs, int64_t units, hipStream_t s) {
+int launch_unpack_batch_int8(const UnpackJob *d_jobs, int njobs, int64_t units, hipStream_t s) {
   if (units <= 0) return CHX_SUCCESS;
   const int64_t gx = units < 65536 ? units : 65536;
   const int64_t gy = (units + gx - 1) / gx;
@@ -488,8 +489,8 @@ int launch_unpack_batch(const UnpackJob *d_jobs, int njobs, int64_t units, hipSt
   return CHX_SUCCESS;
 }
 
-int launch_gemm_batch(const GemmJob *d_jobs, const GemmItem *d_items, const int *d_qbound,
-                      int *d_counters, int nwg, hipStream_t s) {
+int launch_gemm_batch_int8(const GemmJob *d_jobs, const GemmItem *d_items, const int *d_qbound,
+                           int *d_counters, int nwg, hipStream_t s) {
   corr_mfma_batch_kernel<<<dim3((unsigned)nwg), dim3(256), 0, s>>>(d_jobs, d_items, d_qbound,
                                                                    d_counters);
   CHX_HIP(hipGetLastError());

@@ -33,14 +33,15 @@ using Clock = std::chrono::steady_clock;
 
 struct Stream {
   int N, F, T, U;
-  int ntiles, nb, KBf;    // KBf: 16-sample blocks of one frame (padded to 256 samples)
+  int ntiles, nb, KBf;    // layout blocks of one frame: int8 16-sample blocks (frame padded
+                          // to whole rings), fp6 64-sample blocks (group padded instead)
   int64_t Np, NB;
   int64_t integ_frames;   // 0 = no cap
   int64_t submitted = 0, done = 0;
   int32_t *acc = nullptr;
   size_t acc_bytes = 0;
   size_t frame_bytes = 0;
-  size_t plane_bytes = 0;  // int8 planes of one frame
+  size_t plane_bytes = 0;  // planes of one frame alone (a group may pad to whole rings)
   double cmacs_per_frame = 0;
 };
 
@@ -54,10 +55,30 @@ struct Frame {
 
 int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
+// Layout blocks of k frames of a stream side by side: fp6 pads the group to
+// whole rings (zero blocks at its end); int8 frames are ring-aligned already.
+int64_t group_blocks(int fmt, const Stream &s, int64_t k) {
+  return fmt == chx::kPlanesFp6 ? round_up(k * s.KBf, chx::kFp6Ring) : k * s.KBf;
+}
+int64_t group_stages(int fmt, const Stream &s, int64_t k) {
+  const int64_t kb = group_blocks(fmt, s, k);
+  return fmt == chx::kPlanesFp6 ? kb : kb / 4;
+}
+int64_t group_bytes(int fmt, const Stream &s, int64_t k) {
+  return chx::format_plane_bytes(fmt, s.F, s.Np, group_stages(fmt, s, k));
+}
+// One (array, plane, channel) region of a group must stay addressable by a
+// 32-bit buffer descriptor.
+bool group_fits_descriptor(int fmt, const Stream &s, int64_t k) {
+  const int64_t kb = group_blocks(fmt, s, k);
+  return (fmt == chx::kPlanesFp6 ? 32 : 16) * kb * s.Np <= 0x7FFFFFFFLL;
+}
+
 }  // namespace
 
 struct chx_serve {
   chx_serve_config cfg{};
+  int fmt = 1;  // chx::PlaneFormat of the method
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::thread worker;
@@ -111,13 +132,16 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
   for (int64_t sid : order) {
     const std::vector<int> &m = members[sid];
     const Stream &s = *st[m[0]];
-    const int KBt = s.KBf * (int)m.size();
+    const int fmt = S->fmt;
+    const int KBt = (int)group_blocks(fmt, s, (int64_t)m.size());
     uint8_t *planes = S->d_work + off;
-    off += (int64_t)m.size() * (int64_t)s.plane_bytes;
+    uint8_t *planes1 = fmt == kPlanesFp6 ? planes + 64LL * s.F * KBt * s.Np : nullptr;
+    off += group_bytes(fmt, s, (int64_t)m.size());
     for (int k = 0; k < (int)m.size(); k++) {
       UnpackJob u;
       u.in = batch[m[k]].src;
       u.planes = planes;
+      u.planes1 = planes1;
       u.row_bytes = (int64_t)s.F * s.N;
       u.t0 = 0;
       u.t_valid = s.T;
@@ -129,13 +153,24 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
       u.KB_dst = KBt;
       u.kb_dst0 = k * s.KBf;
       ujobs.push_back(u);
-      units += (s.Np + 1023) / 1024 * (int64_t)s.KBf * s.F;
+      units += unpack_units(fmt, u);
     }
-    const int nst = KBt / 4;
+    const int used = s.KBf * (int)m.size();
+    if (KBt > used) {  // ring padding: zero blocks (reads nothing)
+      UnpackJob z = ujobs.back();
+      z.t_valid = 0;
+      z.unit0 = units;
+      z.KB = KBt - used;
+      z.kb_dst0 = used;
+      ujobs.push_back(z);
+      units += unpack_units(fmt, z);
+    }
+    const int nst = (int)group_stages(fmt, s, (int64_t)m.size());
     const int pieces = add_job_items((int)gjobs.size(), s.F, s.ntiles, nst,
                                      S->cfg.max_item_stages, groups);
     GemmJob g;
     g.planes = planes;
+    g.planes1 = planes1;
     g.vis = s.acc;
     g.NB = s.NB;
     g.F = s.F;
@@ -151,7 +186,7 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
   std::vector<GemmItem> items;
   std::vector<int> qbound;
   spread_over_queues(groups, items, qbound);
-  int rc = check_launch_tables(ujobs, gjobs, items, qbound, 1, S->d_work,
+  int rc = check_launch_tables(S->fmt, ujobs, gjobs, items, qbound, 1, S->d_work,
                                (size_t)S->cfg.workspace_bytes);
   if (rc) return rc;
 
@@ -179,9 +214,11 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
   CHX_HIP(hipMemcpyAsync(S->d_tab, S->h_tab, end, hipMemcpyHostToDevice, cs));
   rc = launch_fill_u32((uint32_t *)S->d_counters, 0u, 8, cs);
   if (!rc)
-    rc = launch_unpack_batch((const UnpackJob *)(S->d_tab + a0), (int)ujobs.size(), units, cs);
+    rc = launch_unpack_batch(S->fmt, (const UnpackJob *)(S->d_tab + a0), (int)ujobs.size(),
+                             units, cs);
   if (!rc)
-    rc = launch_gemm_batch((const GemmJob *)(S->d_tab + a1), (const GemmItem *)(S->d_tab + a2),
+    rc = launch_gemm_batch(S->fmt, (const GemmJob *)(S->d_tab + a1),
+                           (const GemmItem *)(S->d_tab + a2),
                            (const int *)(S->d_tab + a3), S->d_counters, S->workgroups, cs);
   if (rc) return rc;
   CHX_HIP(hipEventRecord(S->done_ev, cs));
@@ -215,13 +252,12 @@ void worker_main(chx_serve *S) {
       const Frame &f = S->pending.front();
       if (f.copied && !batch.empty() && hipEventQuery(S->slot_ev[f.slot]) != hipSuccess) break;
       const Stream &s = S->streams.at(f.stream);
-      if (ws + (int64_t)s.plane_bytes > S->cfg.workspace_bytes) break;
-      // one (plane, channel) region of the stream's group stays addressable
-      // by a 32-bit buffer descriptor
-      const int64_t k = per_stream[f.stream] + 1;
-      if (k * s.KBf * s.Np * 16 > 0x7FFFFFFFLL) break;
-      per_stream[f.stream] = k;
-      ws += (int64_t)s.plane_bytes;
+      const int64_t k = per_stream[f.stream];  // frames of this stream taken so far
+      const int64_t grow = group_bytes(S->fmt, s, k + 1) - (k ? group_bytes(S->fmt, s, k) : 0);
+      if (ws + grow > S->cfg.workspace_bytes) break;
+      if (!group_fits_descriptor(S->fmt, s, k + 1)) break;
+      per_stream[f.stream] = k + 1;
+      ws += grow;
       batch.push_back(f);
       st.push_back(&s);
       S->pending.pop_front();
@@ -355,7 +391,11 @@ int chx_serve_create(const chx_serve_config *cfg, chx_serve **out) {
   if (c.max_batch_frames == 0 || c.max_batch_frames > c.frame_slots)
     c.max_batch_frames = c.frame_slots;
   if (c.workspace_bytes == 0) c.workspace_bytes = 2 * (int64_t)c.max_batch_frames * c.slot_bytes;
-  c.max_item_stages = (int32_t)round_up(c.max_item_stages, kMfmaRing);
+  if (c.method == 0) c.method = CHX_METHOD_FP6_MFMA;
+  if (c.method != CHX_METHOD_INT8_MFMA && c.method != CHX_METHOD_FP6_MFMA)
+    return CHX_ERR_INVALID_ARGUMENT;
+  const int fmt = c.method == CHX_METHOD_FP6_MFMA ? kPlanesFp6 : kPlanesInt8;
+  c.max_item_stages = (int32_t)round_up(c.max_item_stages, format_ring(fmt));
   const int64_t fixed = (int64_t)c.frame_slots * c.slot_bytes + c.workspace_bytes;
   if (c.hbm_budget_bytes > 0 && fixed > c.hbm_budget_bytes) return CHX_ERR_BUDGET;
   int ndev = 0;
@@ -368,6 +408,7 @@ int chx_serve_create(const chx_serve_config *cfg, chx_serve **out) {
   chx_serve *S = new (std::nothrow) chx_serve;
   if (!S) return CHX_ERR_OUT_OF_MEMORY;
   S->cfg = c;
+  S->fmt = fmt;
   S->workgroups = std::max(8, cus / 8 * 8);
   hipError_t e = hipStreamCreateWithFlags(&S->copy_s, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->compute_s, hipStreamNonBlocking);
@@ -424,16 +465,18 @@ int chx_serve_open(chx_serve *S, int32_t N, int32_t F, int32_t T, int32_t U,
   s.nb = (int)nb;
   s.Np = nt * kMfmaTile;
   s.NB = NB;
-  s.KBf = (int)(round_up(T, (int64_t)kMfmaStage * kMfmaRing) / 16);
+  const int fmt = S->fmt;
+  s.KBf = fmt == kPlanesFp6 ? (int)((T + kMfmaStage - 1) / kMfmaStage)
+                            : (int)(round_up(T, (int64_t)kMfmaStage * kMfmaRing) / 16);
   s.integ_frames = integration_frames;
   s.acc_bytes = (size_t)F * NB * 2048 * sizeof(int32_t);
   s.frame_bytes = (size_t)T * F * N;
-  s.plane_bytes = (size_t)2 * F * s.Np * s.KBf * 16;
+  s.plane_bytes = (size_t)group_bytes(fmt, s, 1);
   s.cmacs_per_frame = (double)F * T * ((double)N * (N + 1) / 2);
   std::lock_guard<std::mutex> g(S->mu);
   if (S->error) return S->error;
   if ((int64_t)s.plane_bytes > S->cfg.workspace_bytes) return CHX_ERR_BUDGET;
-  if ((int64_t)s.KBf * s.Np * 16 > 0x7FFFFFFFLL) return CHX_ERR_SHAPE;
+  if (!group_fits_descriptor(fmt, s, 1)) return CHX_ERR_SHAPE;
   if (S->cfg.hbm_budget_bytes > 0 &&
       (int64_t)(S->bytes_in_use + s.acc_bytes) > S->cfg.hbm_budget_bytes)
     return CHX_ERR_BUDGET;

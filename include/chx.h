@@ -149,9 +149,10 @@ int chx_stream_synchronize(void *stream);
 
 /* ---- Batch: many correlations of uneven shape in one graph launch. ----
  * No reference counterpart: the reference builds one program per shape and
- * runs one shape per process (main_wrapper.c:309-312).  Every job uses the
- * int8 matrix-core method, and a job's visibilities are bit-identical to
- * what chx_correlate assigns for the same job on its own.  Jobs are fixed
+ * runs one shape per process (main_wrapper.c:309-312).  Every job uses one
+ * matrix-core method (FP6 unless the tuning says int8), and a job's
+ * visibilities are bit-identical to what chx_correlate assigns for the same
+ * job on its own.  Jobs are fixed
  * at creation (device pointers included); each run re-reads the inputs and
  * assigns the outputs.  A batch is not thread-safe; runs of one batch on
  * different streams must not overlap (they share the workspace). */
@@ -169,12 +170,16 @@ typedef struct chx_job_desc {
  * by the autotuner (chime-x_amd/autotune.py). */
 typedef struct chx_batch_tuning {
   int32_t max_item_stages;  /* time extent of one work item, in 64-sample
-                               stages (rounded up to a multiple of 4);
+                               stages (rounded up to a multiple of the
+                               method's ring: 4 int8, 6 FP6);
                                0 = a whole chunk per item */
   int32_t workgroups;       /* persistent workgroups (multiple of 8);
                                0 = one per compute unit */
-  int64_t workspace_bytes;  /* HBM budget of one round's int8 planes;
+  int64_t workspace_bytes;  /* HBM budget of one round's planes;
                                0 = 16 GiB */
+  int32_t method;           /* CHX_METHOD_INT8_MFMA or CHX_METHOD_FP6_MFMA;
+                               0 = FP6 */
+  int32_t reserved;         /* 0 */
 } chx_batch_tuning;
 
 typedef struct chx_batch_info {
@@ -186,6 +191,8 @@ typedef struct chx_batch_info {
   int32_t max_item_stages;
   size_t workspace_bytes;  /* allocated planes workspace */
   int64_t cmacs;           /* sum over jobs of F * T * N(N+1)/2 */
+  int32_t method;          /* matrix-core method used */
+  int32_t reserved;
 } chx_batch_info;
 
 typedef struct chx_batch chx_batch;
@@ -220,20 +227,23 @@ int chx_batch_destroy(chx_batch *batch);
  * the GPU comes free in one launch (frames of many streams together; a
  * stream's frames as one longer chunk).  Frames copy in on their own stream
  * while earlier frames compute.  Results are bit-identical to correlating
- * each stream's frames alone (int8 matrix-core method, 2^32-wrapping int32
+ * each stream's frames alone (matrix-core methods, 2^32-wrapping int32
  * sums).  All entry points are thread-safe. */
 typedef struct chx_serve_config {
   int32_t device;
   int32_t frame_slots;      /* frames in flight at most (copying, queued or
                                computing); a submit needs a free slot */
   int64_t slot_bytes;       /* device buffer per slot: the largest frame */
-  int64_t workspace_bytes;  /* int8 planes of one launch (2 bytes per input
-                               byte when N % 128 == 0 and frame_samples %
-                               256 == 0); 0 = 2 * max_batch_frames *
-                               slot_bytes */
+  int64_t workspace_bytes;  /* planes of one launch (int8: 2 bytes, FP6: 1.5
+                               bytes per input byte when N % 128 == 0 and
+                               frames are whole rings); 0 = 2 *
+                               max_batch_frames * slot_bytes */
   int64_t hbm_budget_bytes; /* slots + workspace + accumulators; 0 = no cap */
   int32_t max_batch_frames; /* frames per launch at most; 0 = frame_slots */
   int32_t max_item_stages;  /* as in chx_batch_tuning; 0 = whole */
+  int32_t method;           /* CHX_METHOD_INT8_MFMA or CHX_METHOD_FP6_MFMA;
+                               0 = FP6 */
+  int32_t reserved;         /* 0 */
 } chx_serve_config;
 
 typedef struct chx_serve_stats {

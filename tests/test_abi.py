@@ -127,10 +127,15 @@ def _batch_create(jobs, tuning=None, n=None):
     ([dict(d_vis=None)], None, chx.ERR_INVALID_ARGUMENT),
     ([dict()], dict(workgroups=12), chx.ERR_INVALID_ARGUMENT),
     ([dict()], dict(max_item_stages=-4), chx.ERR_INVALID_ARGUMENT),
-    # 256 samples of 2 planes x 256 elements are 128 KiB: one byte less fails
-    ([dict()], dict(workspace_bytes=128 * 1024 - 1), chx.ERR_BUDGET),
-    ([dict(num_elements=160), dict(num_elements=32)], dict(workspace_bytes=(512 + 256) * 256 - 1),
-     chx.ERR_BUDGET),
+    ([dict()], dict(method=1), chx.ERR_INVALID_ARGUMENT),
+    # int8: one ring (256 samples) of 2 planes x 256 elements is 128 KiB
+    ([dict()], dict(workspace_bytes=128 * 1024 - 1, method=2), chx.ERR_BUDGET),
+    ([dict(num_elements=160), dict(num_elements=32)],
+     dict(workspace_bytes=(512 + 256) * 256 - 1, method=2), chx.ERR_BUDGET),
+    # fp6 (the default): one ring (384 samples) at 1.5 bytes per value, 144 KiB
+    ([dict()], dict(workspace_bytes=144 * 1024 - 1), chx.ERR_BUDGET),
+    ([dict(num_elements=160), dict(num_elements=32)],
+     dict(workspace_bytes=576 * (256 + 128) - 1, method=3), chx.ERR_BUDGET),
 ])
 def test_batch_validation_is_host_side(jobs, tuning, status):
     assert _batch_create(jobs, tuning) == status
@@ -146,14 +151,16 @@ def test_batch_null_arguments_and_no_device():
     assert L.chx_batch_destroy(None) == chx.ERR_INVALID_ARGUMENT
     if chx.device_count() == 0:
         assert _batch_create([dict(), dict(num_elements=2048, num_freq=8)]) == chx.ERR_NO_DEVICE
-        assert _batch_create([dict()], dict(workspace_bytes=128 * 1024)) == chx.ERR_NO_DEVICE
+        assert _batch_create([dict()], dict(workspace_bytes=128 * 1024, method=2)) == \
+            chx.ERR_NO_DEVICE
+        assert _batch_create([dict()], dict(workspace_bytes=144 * 1024)) == chx.ERR_NO_DEVICE
 
 
 def _fake(jobs):
     return [dict(j, d_input=0x1000, d_vis=0x2000) for j in jobs]
 
 
-def _expected_items(jobs, chunk, max_st):
+def _expected_items(jobs, chunk, max_st, unit):
     """Work items of the plan: per round, per active job, F x tiles x pieces."""
     n, rounds = 0, -(-max(j["time_steps"] for j in jobs) // chunk)
     for r in range(rounds):
@@ -161,27 +168,32 @@ def _expected_items(jobs, chunk, max_st):
             t0 = r * chunk
             if t0 >= j["time_steps"]:
                 continue
-            nst = -(-min(chunk, j["time_steps"] - t0) // 256) * 4
+            nst = -(-min(chunk, j["time_steps"] - t0) // unit) * (unit // 64)
             per = max_st if 0 < max_st < nst else nst
             nt = -(-j["num_elements"] // 128)
             n += j["num_freq"] * nt * (nt + 1) // 2 * -(-nst // per)
     return rounds, n
 
 
+@pytest.mark.parametrize("method", [2, 3])
 @pytest.mark.parametrize("tuning", [None, dict(max_item_stages=64), dict(max_item_stages=6),
                                     dict(workspace_bytes=1 << 30),
                                     dict(workspace_bytes=3 << 30, max_item_stages=512)])
-def test_batch_plan_host_only(tuning):
+def test_batch_plan_host_only(tuning, method):
     import workloads as W
     jobs = W.uneven_batch_jobs()
-    info = chx.batch_plan(_fake(jobs), tuning)
-    t = tuning or {}
-    per_sample = sum(2 * j["num_freq"] * -(-j["num_elements"] // 128) * 128 for j in jobs)
+    t = dict(tuning or {}, method=method)
+    info = chx.batch_plan(_fake(jobs), t)
+    ring = 4 if method == 2 else 6
+    unit = 64 * ring
+    bytes_per_value = 2.0 if method == 2 else 1.5
+    per_unit = sum(int(bytes_per_value * unit) * j["num_freq"] * -(-j["num_elements"] // 128)
+                   * 128 for j in jobs)
     budget = t.get("workspace_bytes", 16 << 30)
     maxT = max(j["time_steps"] for j in jobs)
-    chunk = min(budget // per_sample // 256 * 256, -(-maxT // 256) * 256)
-    max_st = -(-t.get("max_item_stages", 0) // 4) * 4
-    rounds, items = _expected_items(jobs, chunk, max_st)
+    chunk = min(budget // per_unit * unit, -(-maxT // unit) * unit)
+    max_st = -(-t.get("max_item_stages", 0) // ring) * ring
+    rounds, items = _expected_items(jobs, chunk, max_st, unit)
     assert info["chunk_samples"] == chunk
     assert info["rounds"] == rounds and info["items"] == items
     assert info["max_item_stages"] == max_st
@@ -189,6 +201,7 @@ def test_batch_plan_host_only(tuning):
     assert info["cmacs"] == sum(W.cmacs(j["num_elements"], j["num_freq"], j["time_steps"])
                                 for j in jobs)
     assert info["num_jobs"] == 64
+    assert info["method"] == method
 
 
 def test_batch_plan_validates_like_create():
@@ -200,6 +213,9 @@ def test_batch_plan_validates_like_create():
                        dict(workspace_bytes=1000))
     assert e.value.status == chx.ERR_BUDGET
     one = chx.batch_plan(_fake([dict(num_elements=32, num_freq=1, time_steps=1)]))
+    assert one["rounds"] == 1 and one["items"] == 1 and one["chunk_samples"] == 384
+    one = chx.batch_plan(_fake([dict(num_elements=32, num_freq=1, time_steps=1)]),
+                         dict(method=2))
     assert one["rounds"] == 1 and one["items"] == 1 and one["chunk_samples"] == 256
 
 

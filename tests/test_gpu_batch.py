@@ -72,10 +72,11 @@ def mixed_oracle():
     dict(workspace_bytes=MIXED_PER_SAMPLE * 512, max_item_stages=4, workgroups=8),
 ])
 @pytest.mark.parametrize("eager", [True, False])
-def test_mixed_batch_against_oracle(mixed_oracle, tuning, eager):
+@pytest.mark.parametrize("method", [chx.INT8_MFMA, chx.FP6_MFMA])
+def test_mixed_batch_against_oracle(mixed_oracle, tuning, eager, method):
     torch = _torch()
     data, ins, outs, jobs = _mixed(torch)
-    with chx.Batch(jobs, tuning) as b:
+    with chx.Batch(jobs, dict(tuning or {}, method=method)) as b:
         say(f"  batch tuning={tuning} eager={eager} info={b.info()}")
         b.run(eager=eager)
         torch.cuda.synchronize()
@@ -84,23 +85,30 @@ def test_mixed_batch_against_oracle(mixed_oracle, tuning, eager):
                                       err_msg=f"job {k} N={N} F={F} T={T} U={U}")
 
 
-def test_rounds_follow_the_budget():
+@pytest.mark.parametrize("method", [chx.INT8_MFMA, chx.FP6_MFMA])
+def test_rounds_follow_the_budget(method):
     torch = _torch()
     _, _, _, jobs = _mixed(torch)
-    with chx.Batch(jobs, dict(workspace_bytes=MIXED_PER_SAMPLE * 1024)) as b:
+    # one ring of every job: int8 256 samples x 2 bytes, fp6 384 x 1.5 bytes
+    int8 = method == chx.INT8_MFMA
+    unit = 256 if int8 else 384
+    per_unit = MIXED_PER_SAMPLE * unit if int8 else MIXED_PER_SAMPLE * unit * 3 // 4
+    with chx.Batch(jobs, dict(workspace_bytes=MIXED_PER_SAMPLE * 1024, method=method)) as b:
         inf = b.info()
-    assert inf["chunk_samples"] == 1024
-    assert inf["rounds"] == -(-33000 // 1024)
+    chunk = MIXED_PER_SAMPLE * 1024 // per_unit * unit
+    assert inf["chunk_samples"] == chunk
+    assert inf["rounds"] == -(-33000 // chunk)
     assert inf["cmacs"] == sum(W.cmacs(N, F, T) for N, F, T, _ in MIXED)
     with pytest.raises(chx.ChxError) as e:
-        chx.Batch(jobs, dict(workspace_bytes=MIXED_PER_SAMPLE * 255))
+        chx.Batch(jobs, dict(workspace_bytes=per_unit - 1, method=method))
     assert e.value.status == chx.ERR_BUDGET
 
 
-def test_graph_replays_match_eager_and_reread_inputs():
+@pytest.mark.parametrize("method", [chx.INT8_MFMA, chx.FP6_MFMA])
+def test_graph_replays_match_eager_and_reread_inputs(method):
     torch = _torch()
     data, ins, outs, jobs = _mixed(torch, seed=50)
-    with chx.Batch(jobs, dict(max_item_stages=8)) as b:
+    with chx.Batch(jobs, dict(max_item_stages=8, method=method)) as b:
         b.run(eager=True)
         torch.cuda.synchronize()
         eager = [o.clone() for o in outs]
@@ -148,7 +156,8 @@ def _pairs_check(torch, d_in, d_vis, N, U, rng, nsub=16):
     np.testing.assert_array_equal(got, want)
 
 
-def test_uneven_batch_64_jobs_matches_alone_and_oracle():
+@pytest.mark.parametrize("method", [chx.INT8_MFMA, chx.FP6_MFMA])
+def test_uneven_batch_64_jobs_matches_alone_and_oracle(method):
     """The 64-job uneven batch: each job's output from the one graph launch
     equals the job correlated alone (its own Plan) bit for bit, and the
     oracle on a 16-element subset of every job."""
@@ -162,7 +171,7 @@ def test_uneven_batch_64_jobs_matches_alone_and_oracle():
                                 device="cuda"))
         descs.append(dict(j, d_input=ins[-1].data_ptr(), d_vis=outs[-1].data_ptr()))
     total = sum(W.cmacs(j["num_elements"], j["num_freq"], j["time_steps"]) for j in jobs)
-    with chx.Batch(descs) as b:
+    with chx.Batch(descs, dict(method=method)) as b:
         say(f"  uneven batch: {b.info()}")
         b.run()
         torch.cuda.synchronize()
@@ -176,7 +185,7 @@ def test_uneven_batch_64_jobs_matches_alone_and_oracle():
     rng = np.random.default_rng(5)
     for k, j in enumerate(jobs):
         N, F, T, U = j["num_elements"], j["num_freq"], j["time_steps"], 1
-        with chx.Plan(N, F, T, 256, U, chx.INT8_MFMA) as p:
+        with chx.Plan(N, F, T, 256, U, method) as p:
             alone = torch.empty_like(outs[k])
             p.correlate_device(ins[k].data_ptr(), alone.data_ptr(),
                                torch.cuda.current_stream().cuda_stream)

@@ -61,10 +61,11 @@ def _feed(sv, sid, frames, device_tensors=None):
 
 
 @pytest.mark.parametrize("U", [1, 0])
-def test_stream_alone_equals_oracle(U):
+@pytest.mark.parametrize("method", [chx.INT8_MFMA, chx.FP6_MFMA])
+def test_stream_alone_equals_oracle(U, method):
     N, F, T, k = 96, 2, 256, 6
     frames = _frames(N, F, T, k, 300)
-    with chx.Serve(frame_slots=3, slot_bytes=N * F * T) as sv:
+    with chx.Serve(frame_slots=3, slot_bytes=N * F * T, method=method) as sv:
         sid = sv.open(N, F, T, U)
         _feed(sv, sid, frames)
         got = sv.read(sid)
@@ -81,7 +82,8 @@ MIX = [  # (N, F, T per frame, frames, U, device frames?)
 
 
 @pytest.mark.parametrize("max_item_stages", [0, 8])
-def test_streams_in_a_mix_equal_alone(max_item_stages):
+@pytest.mark.parametrize("method", [chx.INT8_MFMA, chx.FP6_MFMA])
+def test_streams_in_a_mix_equal_alone(max_item_stages, method):
     torch = _torch()
     data = [_frames(N, F, T, k, 1000 * j) for j, (N, F, T, k, U, dev) in enumerate(MIX)]
     dev = [[torch.from_numpy(fr).cuda() for fr in frs] if d else None
@@ -91,7 +93,7 @@ def test_streams_in_a_mix_equal_alone(max_item_stages):
 
     def engine():
         return chx.Serve(frame_slots=6, slot_bytes=slot, max_batch_frames=8,
-                         max_item_stages=max_item_stages)
+                         max_item_stages=max_item_stages, method=method)
 
     # alone: one stream per engine run
     alone = []
@@ -127,12 +129,14 @@ def test_streams_in_a_mix_equal_alone(max_item_stages):
                                       err_msg=f"stream {j}")
 
 
-def test_admission_by_budget_and_integration_length():
+@pytest.mark.parametrize("method", [chx.INT8_MFMA, chx.FP6_MFMA])
+def test_admission_by_budget_and_integration_length(method):
     N, F, T = 256, 1, 512
     acc = F * chx.num_blocks(N) * 2048 * 4
     slot = N * F * T
     fixed = 2 * slot + 2 * 2 * slot  # slots + default workspace
-    with chx.Serve(frame_slots=2, slot_bytes=slot, hbm_budget_bytes=fixed + acc) as sv:
+    with chx.Serve(frame_slots=2, slot_bytes=slot, hbm_budget_bytes=fixed + acc,
+                   method=method) as sv:
         a = sv.open(N, F, T, integration_frames=2)
         assert a is not None
         assert sv.open(N, F, T) is None  # no room for a second accumulator
