@@ -14,7 +14,6 @@ without timing anything.
 from __future__ import annotations
 
 import argparse
-import hashlib
 import itertools
 import json
 import os
@@ -28,13 +27,11 @@ import torch  # noqa: E402  (before libchx: one HIP runtime in the process)
 import chx  # noqa: E402
 import workloads as W  # noqa: E402
 
-DEFAULT_CACHE = os.path.join(HERE, "tuning.json")
+DEFAULT_CACHE = chx.TUNING_CACHE
 
 
 def signature(jobs) -> str:
-    shapes = sorted((j["num_elements"], j["num_freq"], j["time_steps"],
-                     j.get("upper_triangle_convention", 1)) for j in jobs)
-    return hashlib.sha1(json.dumps(shapes).encode()).hexdigest()[:16]
+    return chx.job_signature(jobs)
 
 
 def candidates(cus: int):
@@ -83,12 +80,7 @@ def tune(jobs, descs, reps: int = 3, log=print) -> dict:
 
 def load(jobs, path: str = DEFAULT_CACHE) -> dict | None:
     """The cached tuning of this job list's shapes, or None."""
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        cache = json.load(f)
-    ent = cache.get(signature(jobs))
-    return ent["best"]["tuning"] if ent else None
+    return chx.cached_tuning(jobs, path)
 
 
 def save(result: dict, path: str = DEFAULT_CACHE) -> None:
@@ -107,6 +99,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--out", default=DEFAULT_CACHE)
+    ap.add_argument("--replay", type=int, default=0,
+                    help="no tuning: replay the cached shape this many times and report it")
     a = ap.parse_args()
     if not torch.cuda.is_available():
         print("autotune needs a GPU", file=sys.stderr)
@@ -120,6 +114,12 @@ def main() -> int:
                                 device="cuda"))
         descs.append(dict(j, d_input=ins[-1].data_ptr(), d_vis=outs[-1].data_ptr()))
     print(f"uneven batch: {len(jobs)} jobs, signature {signature(jobs)}", flush=True)
+    if a.replay:
+        tuning = load(jobs, a.out)
+        ms = time_batch(descs, tuning, a.replay)
+        total = sum(W.cmacs(j["num_elements"], j["num_freq"], j["time_steps"]) for j in jobs)
+        print(json.dumps(dict(tuning=tuning, ms=ms, tcmacs=total / ms / 1e9)), flush=True)
+        return 0
     res = tune(jobs, descs, a.reps, log=lambda m: print(m, flush=True))
     save(res, a.out)
     print(json.dumps(res["best"]), flush=True)

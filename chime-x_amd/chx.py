@@ -19,6 +19,8 @@ the process would hold two HIP runtimes.
 from __future__ import annotations
 
 import ctypes
+import hashlib
+import json
 import os
 import re
 
@@ -341,16 +343,38 @@ def batch_plan(jobs, tuning: dict | None = None) -> dict:
     return {name: getattr(inf, name) for name, _ in BatchInfo._fields_}
 
 
+TUNING_CACHE = os.path.join(HERE, "tuning.json")
+
+
+def job_signature(jobs) -> str:
+    """Key of a job list's shapes in the autotuner's cache."""
+    shapes = sorted((j["num_elements"], j["num_freq"], j["time_steps"],
+                     j.get("upper_triangle_convention", 1)) for j in jobs)
+    return hashlib.sha1(json.dumps(shapes).encode()).hexdigest()[:16]
+
+
+def cached_tuning(jobs, path: str = TUNING_CACHE):
+    """The autotuned launch shape of this job list's shapes, or None."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        ent = json.load(f).get(job_signature(jobs))
+    return ent["best"]["tuning"] if ent else None
+
+
 class Batch:
-    """Many int8 matrix-core correlations in one captured graph (chx_batch).
+    """Many matrix-core correlations in one captured graph (chx_batch).
 
     jobs: sequence of dicts with num_elements, num_freq, time_steps,
     upper_triangle_convention, d_input, d_vis (device pointers as ints).
-    tuning: None (defaults) or a dict with any of max_item_stages,
+    tuning: None (defaults), "auto" (the autotuner's cached shape for these
+    job shapes, defaults if none), or a dict with any of max_item_stages,
     workgroups, workspace_bytes, method (INT8_MFMA or FP6_MFMA; default FP6).
     """
 
-    def __init__(self, jobs, tuning: dict | None = None, device: int = 0):
+    def __init__(self, jobs, tuning=None, device: int = 0):
+        if tuning == "auto":
+            tuning = cached_tuning(jobs)
         arr = _job_array(jobs)
         self._jobs = arr
         self.tuning = _tuning(tuning)
