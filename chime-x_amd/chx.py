@@ -161,6 +161,8 @@ def lib() -> ctypes.CDLL:
     L.chx_block_maps.argtypes = [i32, vp, vp]
     L.chx_blocks_to_triangle.argtypes = [i32, i32, vp, vp]
     L.chx_blocks_to_triangle_device.argtypes = [i32, i32, vp, vp, vp]
+    L.chx_blocks_to_full.argtypes = [i32, i32, vp, vp]
+    L.chx_blocks_to_full_device.argtypes = [i32, i32, vp, vp, vp]
     L.chx_strerror.argtypes = [ctypes.c_int]
     L.chx_strerror.restype = ctypes.c_char_p
     L.chx_last_error_detail.restype = ctypes.c_char_p
@@ -174,6 +176,8 @@ def lib() -> ctypes.CDLL:
     L.chx_batch_run.argtypes = [vp, i32, vp]
     L.chx_batch_get_info.argtypes = [vp, ctypes.POINTER(BatchInfo)]
     L.chx_batch_destroy.argtypes = [vp]
+    L.chx_generate_device.argtypes = [i32] * 7 + [i64, i32, i32, i32, vp, vp]
+    L.chx_rand_host.argtypes = [i32, i64, i64, vp]
     pi64 = ctypes.POINTER(ctypes.c_int64)
     L.chx_serve_create.argtypes = [ctypes.POINTER(ServeConfig), ctypes.POINTER(vp)]
     L.chx_serve_destroy.argtypes = [vp]
@@ -226,6 +230,20 @@ def blocks_to_triangle(blocks: np.ndarray, N: int) -> np.ndarray:
     tri = np.empty((F, N * (N + 1) // 2, 2), np.int32)
     _check(lib().chx_blocks_to_triangle(F, N, _p(blocks), _p(tri)), "chx_blocks_to_triangle")
     return tri
+
+
+def blocks_to_full(blocks: np.ndarray, N: int) -> np.ndarray:
+    blocks = np.ascontiguousarray(blocks, np.int32)
+    F = blocks.size // (num_blocks(N) * 2048)
+    full = np.empty((F, N, N, 2), np.int32)
+    _check(lib().chx_blocks_to_full(F, N, _p(blocks), _p(full)), "chx_blocks_to_full")
+    return full
+
+
+def blocks_to_full_device(num_freq: int, N: int, d_blocks: int, d_full: int,
+                          stream: int = 0) -> None:
+    _check(lib().chx_blocks_to_full_device(num_freq, N, d_blocks, d_full, stream or None),
+           "chx_blocks_to_full_device")
 
 
 def device_count() -> int:
@@ -511,3 +529,23 @@ class Serve:
 
     def __exit__(self, *exc):
         self.close()
+
+
+GEN_CONSTANT, GEN_RAMP_UP, GEN_RAMP_DOWN, GEN_RANDOM = 1, 2, 3, 4
+
+
+def generate_device(d_data: int, T: int, F: int, N: int, gen_type: int = GEN_RANDOM,
+                    seed: int = 42, default_real: int = 0, default_imag: int = 0,
+                    initial_real: int = 0, initial_imag: int = 0, single_freq: int = -1,
+                    no_repeat: bool = False, stream: int = 0) -> None:
+    """The reference generator (input_generator.c) on the device into d_data."""
+    _check(lib().chx_generate_device(gen_type, seed, default_real, default_imag, initial_real,
+                                     initial_imag, single_freq, T, F, N, int(no_repeat),
+                                     d_data, stream or None), "chx_generate_device")
+
+
+def rand_host(seed: int, k0: int, n: int) -> np.ndarray:
+    """glibc rand() outputs k0 .. k0+n-1 after srand(seed) (jump-ahead)."""
+    out = np.empty(n, np.int32)
+    _check(lib().chx_rand_host(seed, k0, n, _p(out)), "chx_rand_host")
+    return out

@@ -130,6 +130,43 @@ int chx_blocks_to_triangle(int32_t num_freq, int32_t num_elements, const int32_t
   return CHX_SUCCESS;
 }
 
+int chx_blocks_to_full(int32_t num_freq, int32_t num_elements, const int32_t *blocks,
+                       int32_t *full) {
+  if (!blocks || !full || num_freq < 1 || chx_num_blocks(num_elements) < 0)
+    return CHX_ERR_INVALID_ARGUMENT;
+  const int N = num_elements, nb = N / 32;
+  const int64_t NB = chx_num_blocks(N);
+  for (int f = 0; f < num_freq; f++) {
+    int64_t b = 0;
+    int32_t *o = full + (int64_t)f * N * N * 2;
+    for (int by = 0; by < nb; by++)
+      for (int bx = by; bx < nb; bx++, b++) {
+        const int32_t *blk = blocks + ((int64_t)f * NB + b) * 2048;
+        for (int ly = 0; ly < 32; ly++) {
+          const int64_t y = by * 32 + ly;
+          for (int lx = (bx == by ? ly : 0); lx < 32; lx++) {
+            const int64_t x = bx * 32 + lx;
+            const int32_t re = blk[(ly * 32 + lx) * 2], im = blk[(ly * 32 + lx) * 2 + 1];
+            o[(y * N + x) * 2] = re;
+            o[(y * N + x) * 2 + 1] = im;
+            o[(x * N + y) * 2] = re;  // x == y: the same entry again
+            o[(x * N + y) * 2 + 1] = x == y ? im : (int32_t)(0u - (uint32_t)im);
+          }
+        }
+      }
+  }
+  return CHX_SUCCESS;
+}
+
+int chx_blocks_to_full_device(int32_t num_freq, int32_t num_elements, const int32_t *d_blocks,
+                              int32_t *d_full, void *stream) {
+  if (!d_blocks || !d_full || num_freq < 1 || chx_num_blocks(num_elements) < 0)
+    return CHX_ERR_INVALID_ARGUMENT;
+  if ((int64_t)(num_elements / 32) * (num_elements / 32) > 0x7FFFFFFFLL || num_freq > 65535)
+    return CHX_ERR_SHAPE;
+  return launch_blocks_to_full(num_freq, num_elements, d_blocks, d_full, (hipStream_t)stream);
+}
+
 int chx_blocks_to_triangle_device(int32_t num_freq, int32_t num_elements,
                                   const int32_t *d_blocks, int32_t *d_triangle,
                                   void *stream) {

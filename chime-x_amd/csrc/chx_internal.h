@@ -50,6 +50,8 @@ int launch_corr_packed(const chx_plan *p, const uint8_t *in, int32_t *vis,
                        hipStream_t s);
 int launch_blocks_to_triangle(int32_t F, int32_t N, const int32_t *blocks,
                               int32_t *tri, hipStream_t s);
+int launch_blocks_to_full(int32_t F, int32_t N, const int32_t *blocks, int32_t *full,
+                          hipStream_t s);
 
 // int8 matrix-core path (chx_mfma.hip).  mode: 0 assign, 1 add.
 // The plane workspace of one time chunk is at most kWorkspaceBytes.

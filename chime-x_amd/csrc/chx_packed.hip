@@ -453,6 +453,52 @@ int launch_corr_packed(const chx_plan *p, const uint8_t *in, int32_t *vis, hipSt
   return CHX_SUCCESS;
 }
 
+// gpu_data_reorg.c:36-74 on the device: workgroup = one 32x32 tile (ty, tx)
+// of the full matrix of one channel.  The source block -- (ty, tx) on or
+// above the diagonal, (tx, ty) below it -- is staged through LDS so reads
+// and writes are both whole rows; below the diagonal (x < y) an entry is
+// the conjugate of its transpose (imaginary part negated mod 2^32).
+__global__ __launch_bounds__(256) void blocks_to_full_kernel(const int32_t *__restrict__ blocks,
+                                                             int32_t *__restrict__ full,
+                                                             int64_t NB, int N, int nb) {
+  __shared__ int2 tile[32][33];
+  const int tx = blockIdx.x % nb, ty = blockIdx.x / nb, f = blockIdx.y;
+  const int by = ty < tx ? ty : tx, bx = ty < tx ? tx : ty;  // source block, bx >= by
+  const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
+  const int2 *src = (const int2 *)(blocks + ((int64_t)f * NB + b) * 2048);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int e = k * 256 + threadIdx.x;
+    tile[e >> 5][e & 31] = src[e];
+  }
+  __syncthreads();
+  int2 *dst = (int2 *)(full + (int64_t)f * N * N * 2);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int e = k * 256 + threadIdx.x;
+    const int ly = e >> 5, lx = e & 31;
+    const int y = ty * 32 + ly, x = tx * 32 + lx;
+    int2 v;
+    if (x >= y) {
+      v = tile[ly][lx];  // (ty, tx) is the source block itself
+    } else {
+      v = tile[lx][ly];  // transpose of the source block
+      v.y = (int)(0u - (uint32_t)v.y);
+    }
+    dst[(int64_t)y * N + x] = v;
+  }
+}
+
+int launch_blocks_to_full(int32_t F, int32_t N, const int32_t *blocks, int32_t *full,
+                          hipStream_t s) {
+  const int nb = N / 32;
+  const int64_t NB = (int64_t)nb * (nb + 1) / 2;
+  blocks_to_full_kernel<<<dim3((unsigned)(nb * nb), (unsigned)F), dim3(256), 0, s>>>(
+      blocks, full, NB, N, nb);
+  CHX_HIP(hipGetLastError());
+  return CHX_SUCCESS;
+}
+
 int launch_blocks_to_triangle(int32_t F, int32_t N, const int32_t *blocks, int32_t *tri,
                               hipStream_t s) {
   const int nb = N / 32;

@@ -304,6 +304,25 @@ int chx_serve_reset_stats(chx_serve *serve);
 int chx_serve_latencies(chx_serve *serve, int64_t *frame_ids, double *ms, int64_t max,
                         int64_t *count);
 
+/* ---- Test data on the device. ---- */
+
+/* generate_char_data_set (input_generator.c:17-103) on the GPU: fills
+ * d_data ([T][F][N] bytes) byte-identically to the reference generator,
+ * random mode included (libc rand() of glibc after srand(seed), re then im
+ * per element, srand again before every time sample unless no_repeat).
+ * gen_type: 1 constant, 2 ramp up, 3 ramp down, 4 random, other: default
+ * values; single_freq -1 = all channels.  Returns when the data is written
+ * (allocates temporaries). */
+int chx_generate_device(int32_t gen_type, int32_t seed, int32_t default_real,
+                        int32_t default_imag, int32_t initial_real,
+                        int32_t initial_imag, int32_t single_freq, int64_t T,
+                        int32_t F, int32_t N, int32_t no_repeat, uint8_t *d_data,
+                        void *stream);
+
+/* The rand() values k0 .. k0+n-1 after srand(seed), as glibc computes them
+ * (host only; jumps ahead without generating the first k0). */
+int chx_rand_host(int32_t seed, int64_t k0, int64_t n, int32_t *out);
+
 /* ---- Layout helpers (host only, no GPU needed). ---- */
 
 /* NB = (N/32)(N/32+1)/2 (main_wrapper.c:277). Returns -1 on a bad N. */
@@ -316,6 +335,17 @@ int chx_block_maps(int32_t num_elements, uint32_t *x_map, uint32_t *y_map);
  * -> [F][N(N+1)/2][2] packed upper triangle, row-major. */
 int chx_blocks_to_triangle(int32_t num_freq, int32_t num_elements,
                            const int32_t *blocks, int32_t *triangle);
+
+/* reorganize_GPU_to_full_Matrix_for_comparison (gpu_data_reorg.c:36-74):
+ * block order -> the full Hermitian matrix [F][N][N][2], row y, column x;
+ * below the diagonal an entry is the conjugate of its transpose (the
+ * imaginary part negated, wrapping mod 2^32). */
+int chx_blocks_to_full(int32_t num_freq, int32_t num_elements,
+                       const int32_t *blocks, int32_t *full);
+/* The same on the device (the reference's version is host-only and unused
+ * because TRIANGLE is 1, main_wrapper.c:49, 847-855). */
+int chx_blocks_to_full_device(int32_t num_freq, int32_t num_elements,
+                              const int32_t *d_blocks, int32_t *d_full, void *stream);
 
 /* Device version of the same reorg (no reference counterpart: it keeps the
  * visibilities on the GPU for consumers that want the triangle). */

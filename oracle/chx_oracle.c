@@ -242,6 +242,37 @@ void chxo_blocks_to_triangle(int F, int N, const int32_t *blocks, int32_t *tri) 
   }
 }
 
+/* reorganize_GPU_to_full_Matrix_for_comparison (gpu_data_reorg.c:36-74):
+ * every block entry on or above the diagonal (x >= y) goes to full[y][x];
+ * strictly above it (x > y) also to full[x][y] with the imaginary part
+ * negated (:59-62).  Entries below the diagonal of a diagonal block are not
+ * read. */
+void chxo_blocks_to_full(int F, int N, const int32_t *blocks, int32_t *full) {
+  const int nb = N / 32;
+  const int64_t NB = chxo_num_blocks(N);
+  for (int f = 0; f < F; f++) {
+    int64_t b = 0;
+    for (int by = 0; by < nb; by++)
+      for (int bx = by; bx < nb; bx++, b++) {
+        const int32_t *blk = blocks + ((int64_t)f * NB + b) * 2048;
+        for (int ly = 0; ly < 32; ly++)
+          for (int lx = 0; lx < 32; lx++) {
+            const int64_t y = by * 32 + ly, x = bx * 32 + lx;
+            if (x < y) continue;
+            const int32_t re = blk[(ly * 32 + lx) * 2], im = blk[(ly * 32 + lx) * 2 + 1];
+            int32_t *o = full + ((int64_t)f * N * N + y * N + x) * 2;
+            o[0] = re;
+            o[1] = im;
+            if (x > y) {
+              int32_t *t = full + ((int64_t)f * N * N + x * N + y) * 2;
+              t[0] = re;
+              t[1] = (int32_t)(0u - (uint32_t)im);
+            }
+          }
+      }
+  }
+}
+
 /* offset_accumulator.cl:7-49 */
 void chxo_offset_sums(int64_t T, int F, int N, const uint8_t *in, uint32_t *sums) {
   const int64_t row = (int64_t)F * N;

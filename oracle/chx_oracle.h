@@ -69,6 +69,7 @@ void chxo_block_maps(int N, uint32_t *x_map, uint32_t *y_map);
 
 /* gpu_data_reorg.c:76-124: block order -> packed upper triangle. */
 void chxo_blocks_to_triangle(int F, int N, const int32_t *blocks, int32_t *tri);
+void chxo_blocks_to_full(int F, int N, const int32_t *blocks, int32_t *full);
 
 /* offset_accumulator.cl:7-49: sums[F][N][2] = (sum_t R, sum_t I) of the raw
  * nibbles, added into sums (the host zeroes it, main_wrapper.c:671-679). */

@@ -45,6 +45,7 @@ def lib() -> ctypes.CDLL:
         L.chxo_num_blocks.restype = i64
         L.chxo_block_maps.argtypes = [i32, vp, vp]
         L.chxo_blocks_to_triangle.argtypes = [i32, i32, vp, vp]
+        L.chxo_blocks_to_full.argtypes = [i32, i32, vp, vp]
         L.chxo_offset_sums.argtypes = [i64, i32, i32, vp, vp]
         L.chxo_preseed.argtypes = [i32, i32, i64, i32, i32, vp, vp]
         L.chxo_packed_raw.argtypes = [i32, i32, i64, i32, i32, i32, vp, vp]
@@ -114,6 +115,14 @@ def blocks_to_triangle(blocks: np.ndarray, N: int) -> np.ndarray:
     tri = np.empty((F, N * (N + 1) // 2, 2), np.int32)
     lib().chxo_blocks_to_triangle(F, N, _p(blocks), _p(tri))
     return tri
+
+
+def blocks_to_full(blocks: np.ndarray, N: int) -> np.ndarray:
+    blocks = np.ascontiguousarray(blocks, np.int32)
+    F = blocks.size // (num_blocks(N) * 2048)
+    full = np.zeros((F, N, N, 2), np.int32)
+    lib().chxo_blocks_to_full(F, N, _p(blocks), _p(full))
+    return full
 
 
 def offset_sums(data: np.ndarray) -> np.ndarray:

@@ -15,6 +15,7 @@ Reference entry points used (file:line):
   cpu_data_generate_and_correlate                      cpu_corr_test.c:108
   cpu_data_generate_and_correlate_nonstandard_convention cpu_corr_test.c:11
   reorganize_GPU_to_upper_triangle                     gpu_data_reorg.c:76
+  reorganize_GPU_to_full_Matrix_for_comparison         gpu_data_reorg.c:36
   compare_NSquared_correlator_results_data_has_upper_triangle_only
                                                         cpu_corr_test.c:269
 """
@@ -44,6 +45,7 @@ def load_ref():
                  "cpu_data_generate_and_correlate_nonstandard_convention"):
         getattr(L, name).argtypes = [i, i, i, vp] + [i] * 9
     L.reorganize_GPU_to_upper_triangle.argtypes = [i, i, i, i, vp, vp]
+    L.reorganize_GPU_to_full_Matrix_for_comparison.argtypes = [i, i, i, i, vp, vp]
     L.compare_NSquared_correlator_results_data_has_upper_triangle_only.argtypes = [
         vp, vp, i, i, vp, vp, vp, vp, i]
     return L
@@ -101,6 +103,14 @@ def main() -> int:
         tri = np.zeros((F, N * (N + 1) // 2, 2), np.int32)
         L.reorganize_GPU_to_upper_triangle(32, nb, F, N, p(blocks), p(tri))
         out[f"reorg_N{N}_F{F}__tri"] = tri
+
+    # Full Hermitian reorg: signed values (their own index, shifted).
+    for N, F in ((96, 2), (64, 1)):
+        nb = (N // 32) * (N // 32 + 1) // 2
+        blocks = (np.arange(F * nb * 2048, dtype=np.int64) * 37 - 100000).astype(np.int32)
+        full = np.zeros((F, N, N, 2), np.int32)
+        L.reorganize_GPU_to_full_Matrix_for_comparison(32, nb, F, N, p(blocks), p(full))
+        out[f"reorgfull_N{N}_F{F}__full"] = full
 
     # Comparator: planted differences.
     N, F = 64, 2

@@ -253,3 +253,12 @@ def test_serve_validation_is_host_side():
                      (L.chx_serve_latencies, (None, None, None, 0, None)),
                      (L.chx_serve_destroy, (None,))):
         assert fn(*args) == chx.ERR_INVALID_ARGUMENT, fn.__name__
+
+
+def test_blocks_to_full_host_matches_reference(golden):
+    for N, F in ((96, 2), (64, 1)):
+        nb = chx.num_blocks(N)
+        blocks = (np.arange(F * nb * 2048, dtype=np.int64) * 37 - 100000).astype(np.int32)
+        np.testing.assert_array_equal(chx.blocks_to_full(blocks, N),
+                                      golden[f"reorgfull_N{N}_F{F}__full"])
+    assert chx.lib().chx_blocks_to_full(1, 48, None, None) == chx.ERR_INVALID_ARGUMENT

@@ -193,3 +193,31 @@ def test_matrix_long_small_array_multi_chunk(method):
     T, F, N = (1 << 21) + 96, 1, 32
     data = rand(T, F, N, 21)
     np.testing.assert_array_equal(run(data, method, 1, 32), O.correlate_blocks(data, 1))
+
+
+def test_full_matrix_expansion_on_device(golden):
+    torch = _torch()
+    for N, F in ((96, 2), (64, 1)):
+        nb = chx.num_blocks(N)
+        blocks = (np.arange(F * nb * 2048, dtype=np.int64) * 37 - 100000).astype(np.int32)
+        d_b = torch.from_numpy(blocks).cuda()
+        d_full = torch.empty((F, N, N, 2), dtype=torch.int32, device="cuda")
+        chx.blocks_to_full_device(F, N, d_b.data_ptr(), d_full.data_ptr())
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(d_full.cpu().numpy(), golden[f"reorgfull_N{N}_F{F}__full"])
+    # a real correlation at array scale: device expansion == oracle expansion
+    T, F, N = 256, 2, 2048
+    data = rand(T, F, N, 4)
+    with chx.Plan(N, F, T, 256, 1, chx.FP6_MFMA) as p:
+        d_in = torch.from_numpy(data).cuda()
+        d_vis = torch.empty(p.output_shape, dtype=torch.int32, device="cuda")
+        p.correlate_device(d_in.data_ptr(), d_vis.data_ptr())
+        d_full = torch.empty((F, N, N, 2), dtype=torch.int32, device="cuda")
+        chx.blocks_to_full_device(F, N, d_vis.data_ptr(), d_full.data_ptr())
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(d_full.cpu().numpy(),
+                                      O.blocks_to_full(d_vis.cpu().numpy(), N))
+        # Hermitian: full[x][y] == conj(full[y][x])
+        full = d_full.cpu().numpy()
+        np.testing.assert_array_equal(full[..., 0], full[..., 0].transpose(0, 2, 1))
+        np.testing.assert_array_equal(full[..., 1], -full[..., 1].transpose(0, 2, 1))

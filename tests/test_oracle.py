@@ -188,3 +188,30 @@ def test_brute_force_python():
                     want[f, k, 1] += int(v.imag)
                 k += 1
     np.testing.assert_array_equal(O.correlate_triangle(data, 1), want)
+
+
+def test_full_reorg_matches_reference(golden):
+    """chxo_blocks_to_full restates reorganize_GPU_to_full_Matrix_for_comparison
+    (gpu_data_reorg.c:36-74): equal to the reference's output on signed
+    index-valued block buffers."""
+    for N, F in ((96, 2), (64, 1)):
+        nb = O.num_blocks(N)
+        blocks = (np.arange(F * nb * 2048, dtype=np.int64) * 37 - 100000).astype(np.int32)
+        np.testing.assert_array_equal(O.blocks_to_full(blocks, N),
+                                      golden[f"reorgfull_N{N}_F{F}__full"])
+
+
+def test_full_expansion_of_blocks_equals_reference_full_correlator(golden, golden_cases):
+    """Blocks -> Hermitian expansion equals the reference's full N x N CPU
+    correlator (cpu_corr_test.c:108-154 and :11-57) in both conventions."""
+    done = 0
+    for case in golden_cases:
+        if f"{case}__full_U1" not in golden:
+            continue
+        data = golden[f"{case}__input"]
+        N = data.shape[2]
+        for U in (1, 0):
+            np.testing.assert_array_equal(O.blocks_to_full(O.correlate_blocks(data, U), N),
+                                          golden[f"{case}__full_U{U}"], err_msg=f"{case} U={U}")
+        done += 1
+    assert done >= 5
