@@ -36,7 +36,7 @@ def signature(jobs) -> str:
 
 def candidates(cus: int):
     methods = (3, 2)  # FP6, int8 matrix cores (bit-identical results)
-    stages = (0, 510, 252, 126, 66)  # whole chunk, or pieces (rounded to the ring)
+    stages = (0, 1024, 512, 256, 128)  # whole chunk (FP6: <= 2048 stages), or pieces
     wgs = (cus,)
     budgets = (16 << 30, 64 << 30)
     for m, st, wg, ws in itertools.product(methods, stages, wgs, budgets):

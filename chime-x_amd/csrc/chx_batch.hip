@@ -54,10 +54,11 @@ struct chx_batch {
 
 namespace chx {
 
-int add_job_items(int gj, int F, int ntiles, int nst, int max_st, std::vector<ItemGroup> &groups) {
-  const int per = (max_st > 0 && max_st < nst) ? max_st : nst;
+int add_job_items(int fmt, int gj, int F, int ntiles, int nst, int max_st,
+                  std::vector<ItemGroup> &groups) {
+  const int per = format_item_stages(fmt, nst, max_st);
   const int pieces = (nst + per - 1) / per;
-  const int tiles = ntiles * (ntiles + 1) / 2;
+  const int tiles = (int)format_tiles(fmt, ntiles);
   for (int f = 0; f < F; f++) {
     ItemGroup G;
     G.work = 0;
@@ -124,8 +125,9 @@ int check_launch_tables(int fmt, const std::vector<UnpackJob> &ujobs,
     if (w.job < 0 || w.job >= (int)gjobs.size()) return CHX_ERR_HIP;
     const GemmJob &g = gjobs[w.job];
     const int stages = fmt == kPlanesFp6 ? g.KB : g.KB / 4;
-    if (w.f < 0 || w.f >= g.F || w.tile < 0 || w.tile >= g.ntiles * (g.ntiles + 1) / 2 ||
-        w.stage0 < 0 || w.nstages < ring || w.nstages % ring || w.stage0 + w.nstages > stages)
+    if (w.f < 0 || w.f >= g.F || w.tile < 0 || w.tile >= format_tiles(fmt, g.ntiles) ||
+        w.stage0 < 0 || w.nstages < ring || w.nstages % ring || w.stage0 + w.nstages > stages ||
+        w.nstages > format_max_stages(fmt))
       return CHX_ERR_HIP;
   }
   if ((int)qbound.size() != 9 * rounds || qbound.empty() ||
@@ -286,8 +288,8 @@ int plan_tables(chx_batch *b, uint8_t *base, const std::vector<int64_t> &off, Ta
       R.units += unpack_units(b->fmt, u);
 
       // Items of this job: whole chunk per tile, or max_st-stage pieces.
-      const int per = (max_st > 0 && max_st < nst) ? max_st : nst;
-      const int pieces = add_job_items((int)gjobs.size(), F, nt, nst, max_st, groups);
+      const int per = format_item_stages(b->fmt, nst, max_st);
+      const int pieces = add_job_items(b->fmt, (int)gjobs.size(), F, nt, nst, max_st, groups);
       GemmJob g;
       g.planes = planes;
       g.planes1 = planes1;

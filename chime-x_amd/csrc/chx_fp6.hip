@@ -4,13 +4,15 @@
 // E3M2 format (1 sign, 3 exponent bits with bias 3, 2 mantissa bits):
 // 0 -> 0, 1..8 -> 2^e (1 + m/4).  v_mfma_scale_f32_32x32x64_f8f6f4 with
 // both operands E3M2 (and block scales 2^0) issues a 32x32x64 product in the
-// cycles an int8 32x32x32 one takes, and accumulates in FP32.  Every product
-// is an integer of magnitude <= 64 and every partial sum an integer; an
-// accumulator holds at most kChunkStages * 64 samples, |sum| <= 64 * 32640 <
-// 2^21, so all sums are exact in FP32 (24-bit significand) and the flushed
-// int32 visibilities are bit-identical to the int8 path's.  The same four
-// products as the int8 path per 32x32 block:
-//   RE = Ar.Br^T + Ai.Bi^T,  IX = Ai.Br^T,  IY = Ar.Bi^T,  im = +-(IX - IY).
+// cycles an int8 32x32x32 one takes, and accumulates in FP32.  Per 32x32
+// block two accumulators:
+//   RE = Ar.Br^T + Ai.Bi^T,  IM = Ar.Bi^T + (-Ai).Br^T,  im = U ? -IM : IM,
+// where -Ai is Ai with the sign bit of every code flipped (exact).  Every
+// product is an integer of magnitude <= 64, so a sample adds at most 128 to
+// an accumulator; a work item covers at most kFp6MaxStages = 2048 stages of
+// 64 samples, so every partial sum is an integer of magnitude <= 2^24 and
+// exact in FP32: the int32 visibilities are bit-identical to the int8
+// path's.
 //
 // Two passes per time chunk:
 //  1. unpack_f6_kernel (HBM-bound): [time][freq][element] bytes -> E3M2
@@ -20,10 +22,12 @@
 //       P0[plane][F][KB][half][Np][16 B],  P1[plane][F][KB][half][Np][8 B].
 //     A lane's MFMA operand (row l%32, samples 32(l/32)..+31 of the block)
 //     is then one 16-byte and one 8-byte LDS read at lane * 16 / lane * 8.
-//  2. corr_f6_kernel (matrix-core-bound): a workgroup = 4 waves = a
-//     128x128-element tile of one channel, each wave a 64x64 quarter (2x2
-//     blocks x 4 FP32 accumulators = 256 AGPRs), fed by buffer->LDS DMA
-//     through a ring of 6 stages of one 64-sample k-step (24 KB each).
+//  2. corr_f6_kernel: a workgroup = 4 waves = a 128-row x 256-column tile
+//     of one channel, each wave 64 x 128 (2x4 blocks x 2 FP32 accumulators
+//     = 256 AGPRs), fed by buffer->LDS DMA through a ring of 4 stages of one
+//     64-sample k-step (36 KB each), three stages in flight.  The load path
+//     (L2 -> LDS) bounds it: the wide tile moves 25% fewer bytes per product
+//     than a square 128 x 128 one.
 #include "chx_internal.h"
 
 namespace chx {
@@ -37,14 +41,15 @@ typedef unsigned int v2u __attribute__((ext_vector_type(2)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) void lds_void;
 
-constexpr int kTile = kMfmaTile;     // elements per tile side
+constexpr int kTile = kMfmaTile;     // 128: tile rows; a tile has 2 * kTile columns
 constexpr int kK = 64;               // samples per k-step (= per stage)
 constexpr int kRing = kFp6Ring;      // LDS stages in the ring
-constexpr int kChunkStages = 510;    // stages per accumulator flush (multiple of kRing)
-constexpr int kP0Bytes = 16 * 1024;  // [plane 2][group 8][64 lanes][16 B]
-constexpr int kP1Bytes = 8 * 1024;   // [plane 2][group 8][64 lanes][8 B]
+constexpr int kMaxStages = kFp6MaxStages;  // stages of one work item
+constexpr int kGroups = 12;          // 32-element groups per stage: 4 of rows, 8 of columns
+constexpr int kP0Bytes = 2 * kGroups * 1024;  // [plane 2][group 12][64 lanes][16 B]
+constexpr int kP1Bytes = 2 * kGroups * 512;   // [plane 2][group 12][64 lanes][8 B]
 constexpr int kSlotBytes = kP0Bytes + kP1Bytes;
-static_assert(kChunkStages % kRing == 0, "flush chunks hold whole rings");
+static_assert(kMaxStages % kRing == 0, "work items hold whole rings");
 
 // Four nibbles (one per byte, 0-15) -> the E3M2 codes of nibble - 8 (one per
 // byte), by a 16-entry byte table in four words: entries 0-7 (v = -8..-1)
@@ -194,17 +199,17 @@ __global__ __launch_bounds__(256) void unpack_f6_batch_kernel(const UnpackJob *_
 
 // Flush one block's FP32 accumulators (exact integers) into int32
 // visibilities.  Lane l owns column n = l%32 and rows m = 8*(j/4) +
-// 4*(l/32) + j%4 of register j.  mode 0 assigns, 1 adds, 2 adds atomically.
-__device__ __forceinline__ void flush_block_f32(int32_t *blk, const v16f &rea, const v16f &reb,
-                                                const v16f &ix, const v16f &iy, int U,
-                                                int mode, int lane) {
+// 4*(l/32) + j%4 of register j.  re holds Ar.Br + Ai.Bi, im holds Ar.Bi -
+// Ai.Br.  mode 0 assigns, 1 adds, 2 adds atomically.
+__device__ __forceinline__ void flush_block_f32(int32_t *blk, const v16f &re, const v16f &im,
+                                                int U, int mode, int lane) {
   const __amdgpu_buffer_rsrc_t rsrc =
       __builtin_amdgcn_make_buffer_rsrc(blk, (short)0, 2048 * 4, 0x00020000);
   const int voff = ((lane >> 5) * 4 * 32 + (lane & 31)) * 8;
 #pragma unroll
   for (int j = 0; j < 16; j++) {
-    const int32_t vr = (int32_t)(rea[j] + reb[j]);  // |.| < 2^22: exact
-    const int32_t vi = (int32_t)(U ? ix[j] - iy[j] : iy[j] - ix[j]);
+    const int32_t vr = (int32_t)re[j];  // |.| < 2^23: exact
+    const int32_t vi = (int32_t)(U ? -im[j] : im[j]);
     const int soff = (8 * (j >> 2) + (j & 3)) * 32 * 8;
     if (mode == 0) {
       v2i v = {vr, vi};
@@ -227,20 +232,26 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t id, int64_t total) {
   return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
 }
 
-__device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti, int &tj) {
+// Tile tb of one channel: rows [kTile ti, +kTile), columns [kTile ct,
+// +2 kTile), row-major over ti, then ct = ti, ti + 2, ...  Row ti holds
+// ceil((ntiles - ti) / 2) tiles; the last one's second column half lies past
+// the array when ntiles - ti is odd.
+__device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti, int &ct) {
   int r = 0;
-  while (tb >= ntiles - r) {
-    tb -= ntiles - r;
+  while (tb >= (ntiles - r + 1) / 2) {
+    tb -= (ntiles - r + 1) / 2;
     r++;
   }
   ti = r;
-  tj = r + tb;
+  ct = r + 2 * tb;
 }
 
-// Pass 2, one output tile (ti, tj) of channel f over k-steps
-// [st_begin, st_begin + nst) (nst a multiple of kRing).
+// Pass 2, one output tile (rows from kTile ti, columns from kTile ct) of
+// channel f over k-steps [st_begin, st_begin + nst) (nst a multiple of
+// kRing).  Wave w computes rows 64 (w / 2).. x columns 128 (w % 2).. as 2x4
+// blocks of two FP32 accumulators (256 AGPRs).
 __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, const int ti,
-                                             const int tj, const int st_begin, const int nst,
+                                             const int ct, const int st_begin, const int nst,
                                              uint8_t (*lds)[kSlotBytes]) {
   const int F = J.F, Np = J.Np, KB = J.KB, nb = J.nb, U = J.U, mode = J.mode;
   const int64_t NB = J.NB;
@@ -248,11 +259,11 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
-  // Loads: wave w fills plane w/2, groups 4(w&1)..+3: four P0 operands and
-  // two P1 group pairs per stage.  One descriptor per (array, plane,
-  // channel); the lane offset picks element and half, the scalar offset the
-  // k-step.
-  const int lp = wave >> 1;
+  // Loads: wave w fills plane w/2, groups 6(w&1)..+5: six P0 operands and
+  // three P1 group pairs per stage.  Group g < 4 holds rows kTile ti + 32 g,
+  // g >= 4 columns kTile ct + 32 (g - 4); a column half past the array
+  // reloads the first half (its products are never stored).
+  const int lp = wave >> 1, gh = wave & 1;
   const int64_t p0_blk = (int64_t)2 * Np * 16;  // one 64-sample block of one channel
   const int64_t p1_blk = (int64_t)2 * Np * 8;
   const __amdgpu_buffer_rsrc_t r0 = __builtin_amdgcn_make_buffer_rsrc(
@@ -261,148 +272,178 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   const __amdgpu_buffer_rsrc_t r1 = __builtin_amdgcn_make_buffer_rsrc(
       (void *)(J.planes1 + ((int64_t)lp * F + f) * KB * p1_blk), (short)0, (int)(KB * p1_blk),
       0x00020000);
-  int voff0[4], voff1[2];
+  auto group_elem = [&](int g) {
+    int e = g < 4 ? ti * kTile + 32 * g : ct * kTile + 32 * (g - 4);
+    return e < Np ? e : e - kTile;
+  };
+  int voff0[6], voff1[3];
 #pragma unroll
-  for (int gg = 0; gg < 4; gg++) {
-    const int g = 4 * (wave & 1) + gg;
-    const int e0 = (g < 4 ? ti : tj) * kTile + 32 * (g & 3);
-    voff0[gg] = ((lane >> 5) * Np + e0 + (lane & 31)) * 16;
-  }
+  for (int gg = 0; gg < 6; gg++)
+    voff0[gg] = ((lane >> 5) * Np + group_elem(6 * gh + gg) + (lane & 31)) * 16;
 #pragma unroll
-  for (int pp = 0; pp < 2; pp++) {
-    const int g = 4 * (wave & 1) + 2 * pp + (lane >> 5);  // lanes 32-63: the pair's second group
-    const int e0 = (g < 4 ? ti : tj) * kTile + 32 * (g & 3);
+  for (int pp = 0; pp < 3; pp++) {
+    const int g = 6 * gh + 2 * pp + (lane >> 5);  // lanes 32-63: the pair's second group
     const int i = lane & 31;
-    voff1[pp] = ((i >> 4) * Np + e0 + 2 * (i & 15)) * 8;
+    voff1[pp] = ((i >> 4) * Np + group_elem(g) + 2 * (i & 15)) * 8;
   }
   auto issue_stage = [&](int st, int slot) {
     const int kb = st_begin + st;
 #pragma unroll
-    for (int gg = 0; gg < 4; gg++) {
-      const int g = 4 * (wave & 1) + gg;
-      lds_void *dst = (lds_void *)&lds[slot][(lp * 8 + g) * 1024];
+    for (int gg = 0; gg < 6; gg++) {
+      lds_void *dst = (lds_void *)&lds[slot][(lp * kGroups + 6 * gh + gg) * 1024];
       __builtin_amdgcn_raw_ptr_buffer_load_lds(r0, dst, 16, voff0[gg], kb * (int)p0_blk, 0, 0);
     }
 #pragma unroll
-    for (int pp = 0; pp < 2; pp++) {
-      const int g = 4 * (wave & 1) + 2 * pp;
-      lds_void *dst = (lds_void *)&lds[slot][kP0Bytes + (lp * 8 + g) * 512];
+    for (int pp = 0; pp < 3; pp++) {
+      lds_void *dst = (lds_void *)&lds[slot][kP0Bytes + (lp * kGroups + 6 * gh + 2 * pp) * 512];
       __builtin_amdgcn_raw_ptr_buffer_load_lds(r1, dst, 16, voff1[pp], kb * (int)p1_blk, 0, 0);
     }
   };
 
   const int wr = wave >> 1, wc = wave & 1;
-  v16f rea[2][2], reb[2][2], acc_ix[2][2], acc_iy[2][2];
+  v16f re[2][4], im[2][4];
   const v16f zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) rea[i][j] = reb[i][j] = acc_ix[i][j] = acc_iy[i][j] = zero;
-  struct Ops {
-    v8i ar[2], ai[2], br[2], bi[2];
-  };
-  auto load_op = [&](int slot, int plane, int g) -> v8i {
-    const v4i a = *(const v4i *)&lds[slot][(plane * 8 + g) * 1024 + lane * 16];
-    const v2i b = *(const v2i *)&lds[slot][kP0Bytes + (plane * 8 + g) * 512 + lane * 8];
+    for (int j = 0; j < 4; j++) re[i][j] = im[i][j] = zero;
+  // LDS read addresses: one lane/wave base per (operand side, array); the
+  // slot, plane and group enter as immediate offsets.
+  const uint8_t *lds0 = &lds[0][0];
+  const int bA0 = lane * 16 + wr * 2048, bB0 = lane * 16 + 4096 + wc * 4096;
+  const int bA1 = kP0Bytes + lane * 8 + wr * 1024, bB1 = kP0Bytes + lane * 8 + 2048 + wc * 2048;
+  auto load_op = [&](int b0, int b1, int slot, int plane, int k) -> v8i {
+    const v4i a = *(const v4i *)(lds0 + b0 + (slot * kSlotBytes + plane * kGroups * 1024 + k * 1024));
+    const v2i b = *(const v2i *)(lds0 + b1 + (slot * kSlotBytes + plane * kGroups * 512 + k * 512));
     return (v8i){a.x, a.y, a.z, a.w, b.x, b.y, 0, 0};
   };
-  auto read_ops = [&](Ops &o, int slot) {
+  // The sign bit (bit 5) of each of the 32 codes of an operand (16 codes per
+  // 3 words): flipping it negates the value exactly.
+  const v8i sign = {0x20820820, 0x08208208, (int)0x82082082u, 0x20820820, 0x08208208,
+                    (int)0x82082082u, 0, 0};
+  v8i ar[2], ai[2], br[4], bi[4];  // this stage's operands
+  auto read_a = [&](v8i (&xr)[2], v8i (&xi)[2], int slot) {
 #pragma unroll
     for (int i = 0; i < 2; i++) {
-      const int ga = 2 * wr + i, gb = 4 + 2 * wc + i;
-      o.ar[i] = load_op(slot, 0, ga);
-      o.br[i] = load_op(slot, 0, gb);
-      o.ai[i] = load_op(slot, 1, ga);
-      o.bi[i] = load_op(slot, 1, gb);
+      xr[i] = load_op(bA0, bA1, slot, 0, i);
+      xi[i] = load_op(bA0, bA1, slot, 1, i);
     }
   };
-  auto mfma_ops = [&](const Ops &o) {
+  // One stage: for each column group j, its 8 MFMAs, after which that
+  // group's operands of the next stage (slot nslot) replace it; the next
+  // stage's row operands are read during group 0.
+  auto stage_ops = [&](int nslot) {
+    v8i nai[2], nar[2], nxi[2];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 2; i++) nai[i] = ai[i] ^ sign;
+    read_a(nar, nxi, nslot);
 #pragma unroll
-      for (int j = 0; j < 2; j++) {
-        rea[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(o.ar[i], o.br[j], rea[i][j],
-                                                                    3, 3, 0, 127, 0, 127);
-        acc_iy[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-            o.ar[i], o.bi[j], acc_iy[i][j], 3, 3, 0, 127, 0, 127);
-        acc_ix[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-            o.ai[i], o.br[j], acc_ix[i][j], 3, 3, 0, 127, 0, 127);
-        reb[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(o.ai[i], o.bi[j], reb[i][j],
-                                                                    3, 3, 0, 127, 0, 127);
+    for (int j = 0; j < 4; j++) {
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        re[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ar[i], br[j], re[i][j], 3, 3,
+                                                                   0, 127, 0, 127);
+        im[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ar[i], bi[j], im[i][j], 3, 3,
+                                                                   0, 127, 0, 127);
       }
-  };
-  int cur_mode = mode;
-  const int by0 = (ti * kTile + 64 * wr) / 32, bx0 = (tj * kTile + 64 * wc) / 32;
-
-  // Prologue: stages 0..kRing-2 in flight; wait for this wave's stage 0
-  // (6 loads per stage, so (kRing-2)*6 may stay pending), barrier, read it.
 #pragma unroll
-  for (int s = 0; s < kRing - 1; s++) issue_stage(s < nst ? s : nst - 1, s);
-  asm volatile("s_waitcnt vmcnt(24)\n\ts_barrier" ::: "memory");
-  Ops o[2];
-  read_ops(o[0], 0);
+      for (int i = 0; i < 2; i++) {
+        re[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ai[i], bi[j], re[i][j], 3, 3,
+                                                                   0, 127, 0, 127);
+        im[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(nai[i], br[j], im[i][j], 3, 3,
+                                                                   0, 127, 0, 127);
+      }
+      br[j] = load_op(bB0, bB1, nslot, 0, j);
+      bi[j] = load_op(bB0, bB1, nslot, 1, j);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      ar[i] = nar[i];
+      ai[i] = nxi[i];
+    }
+  };
+  const int by0 = (ti * kTile + 64 * wr) / 32, bx0 = (ct * kTile + 128 * wc) / 32;
+
+  // Prologue: stages 0..kRing-2 in flight (9 loads per stage); wait for this
+  // wave's stage 0, barrier, read it.
+#pragma unroll
+  for (int s = 0; s < kRing; s++) issue_stage(s < nst ? s : nst - 1, s);
+  asm volatile("s_waitcnt vmcnt(27)\n\ts_barrier" ::: "memory");
+  read_a(ar, ai, 0);
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    br[j] = load_op(bB0, bB1, 0, 0, j);
+    bi[j] = load_op(bB0, bB1, 0, 1, j);
+  }
 
   // Stage st (slot st % kRing):
-  //   wait for this wave's loads of st+1 (stages st+2..st+kRing-2 may stay
-  //   pending: 3 * 6 loads), barrier: every wave's loads of st+1 landed and
-  //   every wave's reads of the slot of st-1 are done;
-  //   read st+1's operands, two LDS reads per MFMA over st's first 8 MFMAs;
-  //   refill the slot of st-1 with stage st+kRing-1, one DMA per MFMA over
-  //   the remaining ones.
-  for (int c0 = 0; c0 < nst; c0 += kChunkStages) {
-    const int c1 = c0 + kChunkStages < nst ? c0 + kChunkStages : nst;
-    for (int st0 = c0; st0 < c1; st0 += kRing) {
+  //   wait for this wave's loads of st+1 (those of st+2 may stay pending),
+  //   barrier: every wave's loads of st+1 landed; every wave's reads of the
+  //   slot of st-1 are done (they fed st-1's MFMAs);
+  //   MFMAs of st, reading st+1's operands as st's die; refill the slot of
+  //   st-1 with stage st+kRing-1, one DMA per two MFMAs.
+  for (int st0 = 0; st0 < nst; st0 += kRing) {
 #pragma unroll
-      for (int d = 0; d < kRing; d++) {
-        const int st = st0 + d;
-        const int nx = st + kRing - 1;
-        __builtin_amdgcn_sched_barrier(0);
-        asm volatile("s_waitcnt vmcnt(18)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-        __builtin_amdgcn_sched_barrier(0);
-        read_ops(o[(d + 1) & 1], (d + 1) % kRing);
-        mfma_ops(o[d & 1]);
-        issue_stage(nx < nst ? nx : nst - 1, (d + kRing - 1) % kRing);
+    for (int d = 0; d < kRing; d++) {
+      const int st = st0 + d;
+      const int nx = st + kRing;
+      __builtin_amdgcn_sched_barrier(0);
+      asm volatile("s_waitcnt vmcnt(18)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+      stage_ops((d + 1) % kRing);
+      issue_stage(nx < nst ? nx : nst - 1, d);
+      // group 0: 8 MFMAs with the 8 reads of the next row operands
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
-          __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // LDS read
+      for (int i = 0; i < 8; i++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
+      }
+      // groups 1-3: 4 reads (the previous group's next operands) and 3
+      // DMAs over each group's 8 MFMAs
+#pragma unroll
+      for (int g = 0; g < 3; g++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
         }
 #pragma unroll
-        for (int i = 0; i < 6; i++) {
-          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+        for (int i = 0; i < 3; i++) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
           __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM (LDS DMA)
         }
-        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
-        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      }
+      __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);  // the last group's next operands
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  // Every partial sum is an integer of magnitude <= 128 * 64 * kMaxStages =
+  // 2^24: exact in FP32.
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int by = by0 + i, bx = bx0 + j;
+      if (bx >= by && bx < nb) {
+        const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
+        flush_block_f32(vis + ((int64_t)f * NB + b) * 2048, re[i][j], im[i][j], U, mode, lane);
       }
     }
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-      for (int j = 0; j < 2; j++) {
-        const int by = by0 + i, bx = bx0 + j;
-        if (bx >= by && bx < nb) {
-          const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
-          flush_block_f32(vis + ((int64_t)f * NB + b) * 2048, rea[i][j], reb[i][j],
-                          acc_ix[i][j], acc_iy[i][j], U, cur_mode, lane);
-        }
-        rea[i][j] = reb[i][j] = acc_ix[i][j] = acc_iy[i][j] = zero;
-      }
-    if (cur_mode == 0) cur_mode = 1;
-  }
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
-// One plan: grid.x = channels x upper-triangle tiles (XCD-remapped), grid.y
-// = time splits.
+// One plan: grid.x = channels x tiles (XCD-remapped), grid.y = time pieces
+// of stages_per_split stages (the last one the rest of KB).
 __global__ __launch_bounds__(256, 1) void corr_f6_kernel(GemmJob J) {
   __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kSlotBytes];
   const int64_t item = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
-  const int64_t per_f = (int64_t)J.ntiles * (J.ntiles + 1) / 2;
-  int ti, tj;
-  tile_coords((int)(item % per_f), J.ntiles, ti, tj);
-  gemm_tile_f6(J, (int)(item / per_f), ti, tj, blockIdx.y * J.stages_per_split,
-               J.stages_per_split, lds);
+  const int64_t per_f = fp6_tiles(J.ntiles);
+  const int s0 = blockIdx.y * J.stages_per_split;
+  const int ns = J.KB - s0 < J.stages_per_split ? J.KB - s0 : J.stages_per_split;
+  int ti, ct;
+  tile_coords((int)(item % per_f), J.ntiles, ti, ct);
+  gemm_tile_f6(J, (int)(item / per_f), ti, ct, s0, ns, lds);
 }
 
 // Batch: persistent workgroups take work items from 8 per-XCD queues (the
@@ -434,9 +475,9 @@ __global__ __launch_bounds__(256, 1) void corr_f6_batch_kernel(
     if (it < 0) break;
     const GemmItem w = items[it];
     const GemmJob J = jobs[w.job];
-    int ti, tj;
-    tile_coords(w.tile, J.ntiles, ti, tj);
-    gemm_tile_f6(J, w.f, ti, tj, w.stage0, w.nstages, lds);
+    int ti, ct;
+    tile_coords(w.tile, J.ntiles, ti, ct);
+    gemm_tile_f6(J, w.f, ti, ct, w.stage0, w.nstages, lds);
   }
 }
 
@@ -487,8 +528,11 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
   const int N = p->cfg.num_elements, F = p->cfg.num_freq, U = p->cfg.upper_triangle_convention;
   const int64_t T = p->cfg.time_steps;
   const int ntiles = p->ntiles, Np = ntiles * kTile;
-  const int64_t items = (int64_t)F * ntiles * (ntiles + 1) / 2;
+  const int64_t items = (int64_t)F * fp6_tiles(ntiles);
   if (items > 0x7FFFFFFF || !p->d_work) return CHX_ERR_SHAPE;
+  int cus = 0;
+  CHX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->cfg.device));
+  if (cus < 1) cus = 1;
   const int64_t tc = f6_chunk_samples(p);
   const int64_t row_bytes = (int64_t)F * N;
   for (int64_t t0 = 0; t0 < T; t0 += tc) {
@@ -507,12 +551,21 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
                        dim3(256), 0, s>>>(in, p0, p1, N, F, Np, row_bytes, t0, tv, KB);
     CHX_HIP(hipGetLastError());
     if (p->prof) CHX_HIP(hipEventRecord(ev[1], s));
-    // Split time only when the tiles alone cannot fill the chip twice over,
-    // keeping >= 16 stages per split; splits hold whole rings.
-    int split = (int)((512 + items - 1) / items);
-    if (split > nst / 16) split = nst / 16;
-    if (split < 1) split = 1;
-    while ((nst / kRing) % split) split--;
+    // Time splits: the fewest that minimise the rounds of one workgroup per
+    // CU per unit of time (>= 16 stages each), then pieces of whole rings
+    // and at most kMaxStages stages (the last piece may be shorter).
+    int split = 1;
+    double best = (double)((items + cus - 1) / cus);
+    for (int sp = 2; sp <= 64 && nst / sp >= 16; sp++) {
+      const double r = (double)((items * sp + cus - 1) / cus) / sp;
+      if (r < best - 1e-9) {
+        best = r;
+        split = sp;
+      }
+    }
+    int sps = ((nst + split - 1) / split + kRing - 1) / kRing * kRing;
+    if (sps > kMaxStages) sps = kMaxStages;
+    split = (nst + sps - 1) / sps;
     int kmode = (t0 == 0) ? mode : 1;
     if (split > 1) {
       if (kmode == 0) {
@@ -533,7 +586,7 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
     J.nb = p->nb;
     J.U = U;
     J.mode = kmode;
-    J.stages_per_split = nst / split;
+    J.stages_per_split = sps;
     corr_f6_kernel<<<dim3((unsigned)items, (unsigned)split), dim3(256), 0, s>>>(J);
     CHX_HIP(hipGetLastError());
     if (p->prof) CHX_HIP(hipEventRecord(ev[2], s));

@@ -64,8 +64,16 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis,
                      int mode, hipStream_t s);
 
 // FP6 (E3M2) matrix-core path (chx_fp6.hip): same modes, bit-identical
-// results.
-constexpr int kFp6Ring = 6;  // 64-sample stages in the LDS ring
+// results.  Output tiles are 128 rows x 256 columns: row tile ti holds
+// ceil((ntiles - ti) / 2) tiles, the columns starting at row tile ti.  A
+// work item covers at most kFp6MaxStages stages, so that every FP32 partial
+// sum (at most 128 * 64 per stage) stays <= 2^24, where FP32 is exact.
+constexpr int kFp6Ring = 4;           // 64-sample stages in the LDS ring
+constexpr int kFp6MaxStages = 2048;   // stages of one work item
+__host__ __device__ inline int64_t fp6_tiles(int64_t ntiles) {
+  const int64_t m = ntiles / 2;
+  return ntiles % 2 ? (m + 1) * (m + 1) : m * (m + 1);
+}
 size_t fp6_workspace_bytes(const chx_plan *p);
 int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode,
                     hipStream_t s);
@@ -78,6 +86,17 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
 //        64-sample blocks, 1.5 bytes per (element, sample).
 enum PlaneFormat { kPlanesInt8 = 0, kPlanesFp6 = 1 };
 inline int format_ring(int fmt) { return fmt == kPlanesFp6 ? kFp6Ring : kMfmaRing; }
+// output tiles of one channel (int8: 128 x 128 over the upper triangle)
+inline int64_t format_tiles(int fmt, int64_t ntiles) {
+  return fmt == kPlanesFp6 ? fp6_tiles(ntiles) : ntiles * (ntiles + 1) / 2;
+}
+// the most stages one work item may cover (int8: no limit)
+inline int format_max_stages(int fmt) { return fmt == kPlanesFp6 ? kFp6MaxStages : 1 << 30; }
+// stages per work item of an nst-stage chunk cut at max_st (0 = whole)
+inline int format_item_stages(int fmt, int nst, int max_st) {
+  int per = (max_st > 0 && max_st < nst) ? max_st : nst;
+  return per < format_max_stages(fmt) ? per : format_max_stages(fmt);
+}
 // plane bytes of one job chunk of `stages` 64-sample stages (both planes)
 inline int64_t format_plane_bytes(int fmt, int64_t F, int64_t Np, int64_t stages) {
   return fmt == kPlanesFp6 ? 96 * F * Np * stages : 128 * F * Np * stages;
@@ -127,10 +146,12 @@ struct ItemGroup {
   int64_t work;
   std::vector<GemmItem> items;
 };
-// Appends the groups of GEMM job gj (F channels, ntiles tiles per side, nst
-// stages cut into pieces of max_st stages, 0 = whole): one group per channel.
-// Returns the number of pieces per tile.
-int add_job_items(int gj, int F, int ntiles, int nst, int max_st, std::vector<ItemGroup> &groups);
+// Appends the groups of GEMM job gj (F channels, ntiles 128-element tiles
+// per side, nst stages cut into pieces of format_item_stages(fmt, nst,
+// max_st) stages): one group per channel.  Returns the number of pieces per
+// tile.
+int add_job_items(int fmt, int gj, int F, int ntiles, int nst, int max_st,
+                  std::vector<ItemGroup> &groups);
 // Longest-processing-time-first assignment of the groups to 8 queues (long
 // items first inside a queue); appends the items to `items` and the 9 queue
 // bounds to `qbound`.

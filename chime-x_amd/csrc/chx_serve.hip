@@ -166,7 +166,7 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
       units += unpack_units(fmt, z);
     }
     const int nst = (int)group_stages(fmt, s, (int64_t)m.size());
-    const int pieces = add_job_items((int)gjobs.size(), s.F, s.ntiles, nst,
+    const int pieces = add_job_items(fmt, (int)gjobs.size(), s.F, s.ntiles, nst,
                                      S->cfg.max_item_stages, groups);
     GemmJob g;
     g.planes = planes;

@@ -132,10 +132,10 @@ def _batch_create(jobs, tuning=None, n=None):
     ([dict()], dict(workspace_bytes=128 * 1024 - 1, method=2), chx.ERR_BUDGET),
     ([dict(num_elements=160), dict(num_elements=32)],
      dict(workspace_bytes=(512 + 256) * 256 - 1, method=2), chx.ERR_BUDGET),
-    # fp6 (the default): one ring (384 samples) at 1.5 bytes per value, 144 KiB
-    ([dict()], dict(workspace_bytes=144 * 1024 - 1), chx.ERR_BUDGET),
+    # fp6 (the default): one ring (256 samples) at 1.5 bytes per value, 96 KiB
+    ([dict()], dict(workspace_bytes=96 * 1024 - 1), chx.ERR_BUDGET),
     ([dict(num_elements=160), dict(num_elements=32)],
-     dict(workspace_bytes=576 * (256 + 128) - 1, method=3), chx.ERR_BUDGET),
+     dict(workspace_bytes=384 * (256 + 128) - 1, method=3), chx.ERR_BUDGET),
 ])
 def test_batch_validation_is_host_side(jobs, tuning, status):
     assert _batch_create(jobs, tuning) == status
@@ -153,15 +153,24 @@ def test_batch_null_arguments_and_no_device():
         assert _batch_create([dict(), dict(num_elements=2048, num_freq=8)]) == chx.ERR_NO_DEVICE
         assert _batch_create([dict()], dict(workspace_bytes=128 * 1024, method=2)) == \
             chx.ERR_NO_DEVICE
-        assert _batch_create([dict()], dict(workspace_bytes=144 * 1024)) == chx.ERR_NO_DEVICE
+        assert _batch_create([dict()], dict(workspace_bytes=96 * 1024)) == chx.ERR_NO_DEVICE
 
 
 def _fake(jobs):
     return [dict(j, d_input=0x1000, d_vis=0x2000) for j in jobs]
 
 
-def _expected_items(jobs, chunk, max_st, unit):
-    """Work items of the plan: per round, per active job, F x tiles x pieces."""
+def _tiles(nt, method):
+    """Output tiles of one channel: int8 128 x 128 over the upper triangle;
+    fp6 128 x 256, row tile i holding ceil((nt - i) / 2) of them."""
+    if method == 2:
+        return nt * (nt + 1) // 2
+    return sum((nt - i + 1) // 2 for i in range(nt))
+
+
+def _expected_items(jobs, chunk, max_st, unit, method):
+    """Work items of the plan: per round, per active job, F x tiles x pieces
+    (fp6 pieces hold at most 2048 stages)."""
     n, rounds = 0, -(-max(j["time_steps"] for j in jobs) // chunk)
     for r in range(rounds):
         for j in jobs:
@@ -170,9 +179,17 @@ def _expected_items(jobs, chunk, max_st, unit):
                 continue
             nst = -(-min(chunk, j["time_steps"] - t0) // unit) * (unit // 64)
             per = max_st if 0 < max_st < nst else nst
+            if method == 3:
+                per = min(per, 2048)
             nt = -(-j["num_elements"] // 128)
-            n += j["num_freq"] * nt * (nt + 1) // 2 * -(-nst // per)
+            n += j["num_freq"] * _tiles(nt, method) * -(-nst // per)
     return rounds, n
+
+
+def test_fp6_tile_count_closed_form():
+    for nt in range(1, 40):
+        m = nt // 2
+        assert _tiles(nt, 3) == ((m + 1) ** 2 if nt % 2 else m * (m + 1))
 
 
 @pytest.mark.parametrize("method", [2, 3])
@@ -184,7 +201,7 @@ def test_batch_plan_host_only(tuning, method):
     jobs = W.uneven_batch_jobs()
     t = dict(tuning or {}, method=method)
     info = chx.batch_plan(_fake(jobs), t)
-    ring = 4 if method == 2 else 6
+    ring = 4
     unit = 64 * ring
     bytes_per_value = 2.0 if method == 2 else 1.5
     per_unit = sum(int(bytes_per_value * unit) * j["num_freq"] * -(-j["num_elements"] // 128)
@@ -193,7 +210,7 @@ def test_batch_plan_host_only(tuning, method):
     maxT = max(j["time_steps"] for j in jobs)
     chunk = min(budget // per_unit * unit, -(-maxT // unit) * unit)
     max_st = -(-t.get("max_item_stages", 0) // ring) * ring
-    rounds, items = _expected_items(jobs, chunk, max_st, unit)
+    rounds, items = _expected_items(jobs, chunk, max_st, unit, method)
     assert info["chunk_samples"] == chunk
     assert info["rounds"] == rounds and info["items"] == items
     assert info["max_item_stages"] == max_st
@@ -202,6 +219,18 @@ def test_batch_plan_host_only(tuning, method):
                                 for j in jobs)
     assert info["num_jobs"] == 64
     assert info["method"] == method
+
+
+def test_fp6_items_hold_at_most_2048_stages():
+    """A long job is cut into FP6 work items of <= 2048 stages (FP32 partial
+    sums stay <= 2^24, where they are exact); int8 items are not cut."""
+    job = [dict(num_elements=256, num_freq=3, time_steps=5000 * 64)]  # 5000 stages
+    big = dict(workspace_bytes=64 << 30)
+    f6 = chx.batch_plan(_fake(job), dict(big, method=3))
+    assert f6["rounds"] == 1 and f6["chunk_samples"] == 5000 * 64
+    assert f6["items"] == 3 * 2 * 3  # channels x tiles (nt = 2) x ceil(5000 / 2048) pieces
+    i8 = chx.batch_plan(_fake(job), dict(big, method=2))
+    assert i8["items"] == 3 * 3  # channels x tiles (nt = 2: 3 of 128 x 128), whole chunk
 
 
 def test_batch_plan_validates_like_create():
@@ -213,7 +242,7 @@ def test_batch_plan_validates_like_create():
                        dict(workspace_bytes=1000))
     assert e.value.status == chx.ERR_BUDGET
     one = chx.batch_plan(_fake([dict(num_elements=32, num_freq=1, time_steps=1)]))
-    assert one["rounds"] == 1 and one["items"] == 1 and one["chunk_samples"] == 384
+    assert one["rounds"] == 1 and one["items"] == 1 and one["chunk_samples"] == 256
     one = chx.batch_plan(_fake([dict(num_elements=32, num_freq=1, time_steps=1)]),
                          dict(method=2))
     assert one["rounds"] == 1 and one["items"] == 1 and one["chunk_samples"] == 256

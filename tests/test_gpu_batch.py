@@ -89,9 +89,9 @@ def test_mixed_batch_against_oracle(mixed_oracle, tuning, eager, method):
 def test_rounds_follow_the_budget(method):
     torch = _torch()
     _, _, _, jobs = _mixed(torch)
-    # one ring of every job: int8 256 samples x 2 bytes, fp6 384 x 1.5 bytes
+    # one ring of every job: int8 256 samples x 2 bytes, fp6 256 x 1.5 bytes
     int8 = method == chx.INT8_MFMA
-    unit = 256 if int8 else 384
+    unit = 256
     per_unit = MIXED_PER_SAMPLE * unit if int8 else MIXED_PER_SAMPLE * unit * 3 // 4
     with chx.Batch(jobs, dict(workspace_bytes=MIXED_PER_SAMPLE * 1024, method=method)) as b:
         inf = b.info()
@@ -192,3 +192,23 @@ def test_uneven_batch_64_jobs_matches_alone_and_oracle(method):
             torch.cuda.synchronize()
         assert torch.equal(alone, outs[k]), f"job {k} {j}"
         _pairs_check(torch, ins[k], outs[k], N, U, rng)
+
+
+def test_fp6_item_at_the_exact_bound():
+    """Whole-chunk FP6 items of 2048 stages with every sample (-8, -8): each
+    FP32 accumulator reaches 128 * 2048 * 64 = 2^24 (the largest value it
+    holds exactly), and the one-ring rest adds atomically."""
+    torch = _torch()
+    T, F, N = 2048 * 64 + 256, 2, 128
+    x = torch.zeros((T, F, N), dtype=torch.uint8, device="cuda")
+    vis = torch.full((F, chx.num_blocks(N), 32, 32, 2), 3, dtype=torch.int32, device="cuda")
+    job = dict(num_elements=N, num_freq=F, time_steps=T, upper_triangle_convention=1,
+               d_input=x.data_ptr(), d_vis=vis.data_ptr())
+    with chx.Batch([job], dict(method=chx.FP6_MFMA, workspace_bytes=1 << 30)) as b:
+        inf = b.info()
+        say(f"  bound batch: {inf}")
+        assert inf["rounds"] == 1 and inf["items"] == F * 1 * 2
+        b.run()
+        torch.cuda.synchronize()
+    v = vis.cpu().numpy()
+    assert (v[..., 0] == 128 * T).all() and (v[..., 1] == 0).all()

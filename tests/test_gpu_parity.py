@@ -174,6 +174,17 @@ def test_mfma_time_split_and_linearity(method):
     np.testing.assert_array_equal(conj[..., 1], -full[..., 1])
 
 
+def test_fp6_uneven_time_pieces():
+    # 2049 stages of 64 samples plus a ragged tail: FP6 work items hold at
+    # most 2048 stages, so the time is cut into pieces with a shorter last
+    # one, added atomically.
+    T, F, N = 2049 * 64 + 96, 1, 64  # plans take T % 32 == 0
+    data = rand(T, F, N, 2049)
+    got = run(data, chx.FP6_MFMA, 1, 32)
+    np.testing.assert_array_equal(got, run(data, chx.INT8_MFMA, 1, 32))
+    np.testing.assert_array_equal(got, O.correlate_blocks(data, 1))
+
+
 def test_methods_agree_on_array_scale_channel_slice():
     # N=2048 (config 2's array) on a T slice: all three methods bit-exact
     # against each other and against the oracle.
