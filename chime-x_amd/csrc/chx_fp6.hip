@@ -541,16 +541,6 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
     const int KB = nst;  // one 64-sample block per stage
     uint8_t *p0 = p->d_work;
     uint8_t *p1 = p0 + (int64_t)2 * F * KB * 2 * Np * 16;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    if (p->prof) {
-      for (hipEvent_t &e : ev) CHX_HIP(hipEventCreate(&e));
-      p->prof_ev->insert(p->prof_ev->end(), ev, ev + 3);
-      CHX_HIP(hipEventRecord(ev[0], s));
-    }
-    unpack_f6_kernel<<<dim3((unsigned)((Np + 1023) / 1024), (unsigned)(2 * KB), (unsigned)F),
-                       dim3(256), 0, s>>>(in, p0, p1, N, F, Np, row_bytes, t0, tv, KB);
-    CHX_HIP(hipGetLastError());
-    if (p->prof) CHX_HIP(hipEventRecord(ev[1], s));
     // Time splits: the fewest that minimise the rounds of one workgroup per
     // CU per unit of time (>= 16 stages each), then pieces of whole rings
     // and at most kMaxStages stages (the last piece may be shorter).
@@ -567,13 +557,23 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
     if (sps > kMaxStages) sps = kMaxStages;
     split = (nst + sps - 1) / sps;
     int kmode = (t0 == 0) ? mode : 1;
-    if (split > 1) {
+    if (split > 1) {  // pieces add atomically into zeroed (or accumulated) output
       if (kmode == 0) {
         int rc = launch_fill_u32((uint32_t *)vis, 0u, (int64_t)F * p->NB * 2048, s);
         if (rc) return rc;
       }
       kmode = 2;
     }
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    if (p->prof) {
+      for (hipEvent_t &e : ev) CHX_HIP(hipEventCreate(&e));
+      p->prof_ev->insert(p->prof_ev->end(), ev, ev + 3);
+      CHX_HIP(hipEventRecord(ev[0], s));
+    }
+    unpack_f6_kernel<<<dim3((unsigned)((Np + 1023) / 1024), (unsigned)(2 * KB), (unsigned)F),
+                       dim3(256), 0, s>>>(in, p0, p1, N, F, Np, row_bytes, t0, tv, KB);
+    CHX_HIP(hipGetLastError());
+    if (p->prof) CHX_HIP(hipEventRecord(ev[1], s));
     GemmJob J;
     J.planes = p0;
     J.planes1 = p1;
