@@ -163,19 +163,22 @@ __device__ __forceinline__ void unpack_f6_unit(const uint8_t *__restrict__ in,
 // half of a 64-sample block of one channel.  Thread: 32 word loads (one per
 // sample row; 256 contiguous bytes per wave and row), 8 byte transposes,
 // then per plane: codes, bit packing, LDS staging, and full-run global
-// stores (1 KB of P0 or 512 B of P1 per wave instruction).
+// stores (1 KB of P0 or 512 B of P1 per wave instruction).  Grid: x =
+// channel, y = element group, z = (block, half), so that workgroups running
+// together read whole input rows.
 __global__ __launch_bounds__(256) void unpack_f6_kernel(
     const uint8_t *__restrict__ in, uint8_t *__restrict__ p0, uint8_t *__restrict__ p1, int N,
     int F, int Np, int64_t row_bytes, int64_t t0, int64_t t_valid, int KB) {
   __shared__ __attribute__((aligned(16))) v4u s0[1024 + 64];  // [element] P0, padded
   __shared__ __attribute__((aligned(16))) v2u s1[1024 + 64];  // [element] P1, padded
-  unpack_f6_unit(in, p0, p1, N, F, Np, row_bytes, t0, t_valid, KB, blockIdx.x, blockIdx.y >> 1,
-                 0, blockIdx.y & 1, blockIdx.z, s0, s1);
+  unpack_f6_unit(in, p0, p1, N, F, Np, row_bytes, t0, t_valid, KB, blockIdx.y, blockIdx.z >> 1,
+                 0, blockIdx.z & 1, blockIdx.x, s0, s1);
 }
 
 // Batch: one launch unpacks the chunks of many jobs; workgroup g belongs to
 // the job j with jobs[j].unit0 <= g < jobs[j+1].unit0 (binary search); its
-// unit within the job is (channel, block, half, element group).
+// unit within the job is ((block, half), element group, channel), channel
+// fastest.
 __global__ __launch_bounds__(256) void unpack_f6_batch_kernel(const UnpackJob *__restrict__ jobs,
                                                               int njobs) {
   __shared__ __attribute__((aligned(16))) v4u s0[1024 + 64];
@@ -190,9 +193,9 @@ __global__ __launch_bounds__(256) void unpack_f6_batch_kernel(const UnpackJob *_
   const int64_t u = g - J.unit0;
   const int nq = (J.Np + 1023) / 1024;
   if (u >= (int64_t)nq * 2 * J.KB * J.F) return;  // grid padding (uniform per workgroup)
-  const int qb = (int)(u % nq);
-  const int hk = (int)((u / nq) % (2 * J.KB));  // (block, half) within the job
-  const int f = (int)(u / ((int64_t)nq * 2 * J.KB));
+  const int f = (int)(u % J.F);  // channel fastest: neighbours read whole rows
+  const int qb = (int)((u / J.F) % nq);
+  const int hk = (int)(u / ((int64_t)J.F * nq));  // (block, half) within the job
   unpack_f6_unit(J.in, J.planes, J.planes1, J.N, J.F, J.Np, J.row_bytes, J.t0, J.t_valid,
                  J.KB_dst, qb, J.kb_dst0 + (hk >> 1), J.kb_dst0, hk & 1, f, s0, s1);
 }
@@ -570,7 +573,7 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
       p->prof_ev->insert(p->prof_ev->end(), ev, ev + 3);
       CHX_HIP(hipEventRecord(ev[0], s));
     }
-    unpack_f6_kernel<<<dim3((unsigned)((Np + 1023) / 1024), (unsigned)(2 * KB), (unsigned)F),
+    unpack_f6_kernel<<<dim3((unsigned)F, (unsigned)((Np + 1023) / 1024), (unsigned)(2 * KB)),
                        dim3(256), 0, s>>>(in, p0, p1, N, F, Np, row_bytes, t0, tv, KB);
     CHX_HIP(hipGetLastError());
     if (p->prof) CHX_HIP(hipEventRecord(ev[1], s));

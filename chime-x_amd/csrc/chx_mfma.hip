@@ -164,17 +164,20 @@ __device__ __forceinline__ void unpack_unit(const uint8_t *__restrict__ in,
   }
 }
 
+// Grid: x = channel, y = element group, z = block, so that workgroups
+// running together read whole input rows.
 __global__ __launch_bounds__(256) void unpack_planes_kernel(
     const uint8_t *__restrict__ in, uint8_t *__restrict__ out_planes, int N, int F, int Np,
     int64_t row_bytes, int64_t t0, int64_t t_valid, int KB) {
   __shared__ __attribute__((aligned(16))) v4u stage[2][1024 + 64];  // [plane][element], padded
-  unpack_unit(in, out_planes, N, F, Np, row_bytes, t0, t_valid, KB, blockIdx.x, blockIdx.y, 0,
-              blockIdx.z, stage);
+  unpack_unit(in, out_planes, N, F, Np, row_bytes, t0, t_valid, KB, blockIdx.y, blockIdx.z, 0,
+              blockIdx.x, stage);
 }
 
 // Batch: one launch unpacks the chunks of many jobs.  Workgroup g belongs
 // to the job j with jobs[j].unit0 <= g < jobs[j+1].unit0 (binary search);
-// its unit index within the job is (channel, block, element group).  A job
+// its unit index within the job is (block, element group, channel), channel
+// fastest so that neighbours read whole input rows.  A job
 // writes blocks [kb_dst0, kb_dst0 + KB) of a destination laid out with
 // KB_dst blocks per channel (several frames of one stream side by side).
 __global__ __launch_bounds__(256) void unpack_batch_kernel(const UnpackJob *__restrict__ jobs,
@@ -190,9 +193,9 @@ __global__ __launch_bounds__(256) void unpack_batch_kernel(const UnpackJob *__re
   const int64_t u = g - J.unit0;
   const int nq = (J.Np + 1023) / 1024;
   if (u >= (int64_t)nq * J.KB * J.F) return;  // grid padding (uniform per workgroup)
-  const int qb = (int)(u % nq);
-  const int kb = (int)((u / nq) % J.KB);
-  const int f = (int)(u / ((int64_t)nq * J.KB));
+  const int f = (int)(u % J.F);
+  const int qb = (int)((u / J.F) % nq);
+  const int kb = (int)(u / ((int64_t)J.F * nq));
   unpack_unit(J.in, J.planes, J.N, J.F, J.Np, J.row_bytes, J.t0, J.t_valid, J.KB_dst, qb,
               J.kb_dst0 + kb, J.kb_dst0, f, stage);
 }
@@ -474,7 +477,7 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
 int launch_unpack_planes(const uint8_t *in, uint8_t *planes, int N, int F, int Np,
                          int64_t row_bytes, int64_t t0, int64_t t_valid, int KB,
                          hipStream_t s) {
-  unpack_planes_kernel<<<dim3((unsigned)((Np + 1023) / 1024), (unsigned)KB, (unsigned)F),
+  unpack_planes_kernel<<<dim3((unsigned)F, (unsigned)((Np + 1023) / 1024), (unsigned)KB),
                          dim3(256), 0, s>>>(in, planes, N, F, Np, row_bytes, t0, t_valid, KB);
   CHX_HIP(hipGetLastError());
   return CHX_SUCCESS;
