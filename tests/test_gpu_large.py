@@ -1,5 +1,6 @@
 """The large/long configuration on the GPU: N=4096, F=32, 2^20 samples
-integrated in int32 (128 GiB of input resident in HBM, 16 plane chunks).
+integrated in int32 (128 GiB of input resident in HBM, many plane chunks),
+for both matrix-core methods.
 
 Checks that hold at full size without a full CPU correlation:
   * saturation: every byte 0x00 = (-8,-8) gives re = 128 * 2^20 =
@@ -30,17 +31,25 @@ def say(*a):
 
 
 @pytest.fixture(scope="module")
-def big():
+def buffers():
     torch = pytest.importorskip("torch")
     if not torch.cuda.is_available():
         pytest.skip("torch sees no GPU")
     x = torch.empty((T, F, N), dtype=torch.uint8, device="cuda")
     out = torch.empty((F, chx.num_blocks(N), 32, 32, 2), dtype=torch.int32, device="cuda")
-    plan = chx.Plan(N, F, T, 256, 1, chx.INT8_MFMA)
-    yield torch, x, out, plan
-    plan.close()
+    yield torch, x, out
     del x, out
     torch.cuda.empty_cache()
+
+
+@pytest.fixture(scope="module", params=[chx.INT8_MFMA, chx.FP6_MFMA],
+                ids=["int8_mfma", "fp6_mfma"])
+def big(buffers, request):
+    torch, x, out = buffers
+    plan = chx.Plan(N, F, T, 256, 1, request.param)
+    say(f"  method {chx.METHOD_NAMES[request.param]}")
+    yield torch, x, out, plan
+    plan.close()
 
 
 def _run(torch, plan, x, out):
@@ -100,7 +109,7 @@ def test_uniform_subset_oracle_and_time_linearity(big):
     # linearity in time: two half-length plans add up to the whole (mod 2^32)
     half = T // 2
     a = torch.empty_like(out)
-    with chx.Plan(N, F, half, 256, 1, chx.INT8_MFMA) as p:
+    with chx.Plan(N, F, half, 256, 1, plan.cfg.kernel_batch) as p:
         p.correlate_device(x.data_ptr(), a.data_ptr())
         p.corr(x[half:].data_ptr(), a.data_ptr())  # adds the second half
         torch.cuda.synchronize()
