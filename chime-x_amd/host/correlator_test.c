@@ -8,9 +8,9 @@
  *   -c check_results, -v verbose, -g gen_type, -r random_seed,
  *   -p no_repeat_random, -q generate_frequency, -x/-y default_real/imag,
  *   -X/-Y initial_real/imag, -k kernel_batch (0 conference, 1 highly
- *   packed, 2 int8 matrix cores), -h help.
+ *   packed, 2 int8 matrix cores, 3 FP6 matrix cores), -h help.
  * Deliberate differences: -Y sets initial_imaginary (the reference writes
- * initial_real, main_wrapper.c:203-205); -k also accepts 2; byte counts are
+ * initial_real, main_wrapper.c:203-205); -k also accepts 2 and 3; byte counts are
  * 64-bit (main_wrapper.c:403); the data generator and the --check_results
  * CPU correlator below are this program's own restatements of
  * input_generator.c:17-103 and cpu_corr_test.c:59-106/156-203.
@@ -38,7 +38,7 @@ static void print_help(void) {
       "  --help (-h)\n"
       "  --device (-d) [n]                 Default: 0.\n"
       "  --iterations (-i) [n]             Default: 100.\n"
-      "  --time_accum (-t) [n]             Default: 256. [8,288] step 8 for -k 0, [8,2272] for -k 1, any multiple of 8 for -k 2.\n"
+      "  --time_accum (-t) [n]             Default: 256. [8,288] step 8 for -k 0, [8,2272] for -k 1, any multiple of 8 for -k 2 and 3.\n"
       "  --time_steps (-T) [n]             Default: 128*time_accum. Multiple of 32 and of time_accum.\n"
       "  --num_freq (-f) [n]               Default: 1.\n"
       "  --num_elem (-e) [n]               Default: 2048. Multiple of 32.\n"
@@ -52,7 +52,7 @@ static void print_help(void) {
       "  --generate_frequency (-q) [n]     Default: -1 (all). Other channels get the default value.\n"
       "  --default_real (-x) / --default_imaginary (-y) [n]  Default: 0, range [-8,7].\n"
       "  --initial_real (-X) / --initial_imaginary (-Y) [n]  Default: 0, range [-8,7].\n"
-      "  --kernel_batch (-k) [n]           Default: 0. 0 conference, 1 highly packed, 2 int8 matrix cores.\n");
+      "  --kernel_batch (-k) [n]           Default: 0. 0 conference, 1 highly packed, 2 int8 matrix cores, 3 FP6 matrix cores.\n");
 }
 
 static int clip(int v, int off, int lo, int hi) {
@@ -188,7 +188,7 @@ int main(int argc, char **argv) {
       case 'Y': iim = atoi(optarg); break;
       case 'k':
         kernel_batch = atoi(optarg);
-        if (kernel_batch < 0 || kernel_batch > 2) {
+        if (kernel_batch < 0 || kernel_batch > 3) {
           printf("Invalid parameter for kernel_batch.  See help for options\n");
           print_help();
           return -1;
