@@ -436,11 +436,12 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
-// One plan: grid.x = channels x tiles (XCD-remapped), grid.y = time pieces
-// of stages_per_split stages (the last one the rest of KB).
-__global__ __launch_bounds__(256, 1) void corr_f6_kernel(GemmJob J) {
+// One plan: grid.x = items item0.. of channels x tiles (XCD-remapped),
+// grid.y = time pieces of stages_per_split stages (the last one the rest of
+// KB).
+__global__ __launch_bounds__(256, 1) void corr_f6_kernel(GemmJob J, int item0) {
   __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kSlotBytes];
-  const int64_t item = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
+  const int64_t item = item0 + xcd_remap(blockIdx.x, (int64_t)gridDim.x);
   const int64_t per_f = fp6_tiles(J.ntiles);
   const int s0 = blockIdx.y * J.stages_per_split;
   const int ns = J.KB - s0 < J.stages_per_split ? J.KB - s0 : J.stages_per_split;
@@ -544,28 +545,36 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
     const int KB = nst;  // one 64-sample block per stage
     uint8_t *p0 = p->d_work;
     uint8_t *p1 = p0 + (int64_t)2 * F * KB * 2 * Np * 16;
-    // Time splits: the fewest that minimise the rounds of one workgroup per
-    // CU per unit of time (>= 16 stages each), then pieces of whole rings
-    // and at most kMaxStages stages (the last piece may be shorter).
+    // Launch shape.  Whole rounds of one workgroup per CU take whole items
+    // (one piece of the chunk, plain stores); the items of a last partial
+    // round are cut in time into the fewest pieces that minimise its length
+    // and add atomically into zeroed output.  A chunk longer than kMaxStages
+    // cuts every item into pieces of at most kMaxStages stages (atomic).
+    int64_t head = 0;  // items of the whole rounds
     int split = 1;
-    double best = (double)((items + cus - 1) / cus);
-    for (int sp = 2; sp <= 64 && nst / sp >= 16; sp++) {
-      const double r = (double)((items * sp + cus - 1) / cus) / sp;
-      if (r < best - 1e-9) {
-        best = r;
-        split = sp;
+    if (nst <= kMaxStages) head = items / cus * cus;
+    const int64_t tail = items - head;
+    if (tail > 0) {
+      double best = (double)((tail + cus - 1) / cus);
+      for (int sp = 2; sp <= 64 && nst / sp >= 16; sp++) {
+        const double r = (double)((tail * sp + cus - 1) / cus) / sp;
+        if (r < best - 1e-9) {
+          best = r;
+          split = sp;
+        }
       }
     }
     int sps = ((nst + split - 1) / split + kRing - 1) / kRing * kRing;
     if (sps > kMaxStages) sps = kMaxStages;
     split = (nst + sps - 1) / sps;
-    int kmode = (t0 == 0) ? mode : 1;
-    if (split > 1) {  // pieces add atomically into zeroed (or accumulated) output
-      if (kmode == 0) {
-        int rc = launch_fill_u32((uint32_t *)vis, 0u, (int64_t)F * p->NB * 2048, s);
-        if (rc) return rc;
-      }
-      kmode = 2;
+    const int kmode = (t0 == 0) ? mode : 1;
+    const int tmode = split > 1 ? 2 : kmode;  // the tail's pieces add atomically
+    if (tail > 0 && split > 1 && kmode == 0) {
+      // zero the channels the tail touches (before the head's stores)
+      const int64_t f0 = head / fp6_tiles(ntiles);
+      int rc = launch_fill_u32((uint32_t *)vis + f0 * p->NB * 2048, 0u,
+                               (int64_t)(F - f0) * p->NB * 2048, s);
+      if (rc) return rc;
     }
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     if (p->prof) {
@@ -588,10 +597,18 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
     J.ntiles = ntiles;
     J.nb = p->nb;
     J.U = U;
-    J.mode = kmode;
-    J.stages_per_split = sps;
-    corr_f6_kernel<<<dim3((unsigned)items, (unsigned)split), dim3(256), 0, s>>>(J);
-    CHX_HIP(hipGetLastError());
+    if (head > 0) {
+      J.mode = kmode;
+      J.stages_per_split = nst;
+      corr_f6_kernel<<<dim3((unsigned)head, 1u), dim3(256), 0, s>>>(J, 0);
+      CHX_HIP(hipGetLastError());
+    }
+    if (tail > 0) {
+      J.mode = tmode;
+      J.stages_per_split = sps;
+      corr_f6_kernel<<<dim3((unsigned)tail, (unsigned)split), dim3(256), 0, s>>>(J, (int)head);
+      CHX_HIP(hipGetLastError());
+    }
     if (p->prof) CHX_HIP(hipEventRecord(ev[2], s));
   }
   return CHX_SUCCESS;
