@@ -436,15 +436,30 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
-// One plan: grid.x = items item0.. of channels x tiles (XCD-remapped),
-// grid.y = time pieces of stages_per_split stages (the last one the rest of
-// KB).
-__global__ __launch_bounds__(256, 1) void corr_f6_kernel(GemmJob J, int item0) {
+// One plan chunk: items = channels x tiles.  Workgroups 0..head-1 take
+// items 0..head-1 whole (all KB stages, mode J.mode); the rest take the
+// tail items head.. cut into time pieces of J.stages_per_split stages (the
+// last piece the rest of KB), mode tmode.  Each range is XCD-remapped (with
+// a CU count that is a multiple of 8, head is too, and both ranges keep
+// blockIdx % 8 = XCD).
+__global__ __launch_bounds__(256, 1) void corr_f6_kernel(GemmJob J, int head, int tail,
+                                                         int tmode) {
   __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kSlotBytes];
-  const int64_t item = item0 + xcd_remap(blockIdx.x, (int64_t)gridDim.x);
   const int64_t per_f = fp6_tiles(J.ntiles);
-  const int s0 = blockIdx.y * J.stages_per_split;
-  const int ns = J.KB - s0 < J.stages_per_split ? J.KB - s0 : J.stages_per_split;
+  int64_t item;
+  int s0, ns;
+  if ((int)blockIdx.x < head) {
+    item = xcd_remap(blockIdx.x, head);
+    s0 = 0;
+    ns = J.KB;
+  } else {
+    const int64_t pieces = (int64_t)gridDim.x - head;
+    const int64_t r = xcd_remap((int64_t)blockIdx.x - head, pieces);
+    item = head + r % tail;
+    s0 = (int)(r / tail) * J.stages_per_split;
+    ns = J.KB - s0 < J.stages_per_split ? J.KB - s0 : J.stages_per_split;
+    J.mode = tmode;
+  }
   int ti, ct;
   tile_coords((int)(item % per_f), J.ntiles, ti, ct);
   gemm_tile_f6(J, (int)(item / per_f), ti, ct, s0, ns, lds);
@@ -545,11 +560,12 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
     const int KB = nst;  // one 64-sample block per stage
     uint8_t *p0 = p->d_work;
     uint8_t *p1 = p0 + (int64_t)2 * F * KB * 2 * Np * 16;
-    // Launch shape.  Whole rounds of one workgroup per CU take whole items
-    // (one piece of the chunk, plain stores); the items of a last partial
-    // round are cut in time into the fewest pieces that minimise its length
-    // and add atomically into zeroed output.  A chunk longer than kMaxStages
-    // cuts every item into pieces of at most kMaxStages stages (atomic).
+    // Launch shape (one dispatch).  Whole rounds of one workgroup per CU
+    // take whole items (the chunk in one piece, plain stores); the items of
+    // a last partial round are cut in time into the fewest pieces that
+    // minimise its length and add atomically into zeroed output.  A chunk
+    // longer than kMaxStages cuts every item into pieces of at most
+    // kMaxStages stages (atomic).
     int64_t head = 0;  // items of the whole rounds
     int split = 1;
     if (nst <= kMaxStages) head = items / cus * cus;
@@ -597,18 +613,12 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
     J.ntiles = ntiles;
     J.nb = p->nb;
     J.U = U;
-    if (head > 0) {
-      J.mode = kmode;
-      J.stages_per_split = nst;
-      corr_f6_kernel<<<dim3((unsigned)head, 1u), dim3(256), 0, s>>>(J, 0);
-      CHX_HIP(hipGetLastError());
-    }
-    if (tail > 0) {
-      J.mode = tmode;
-      J.stages_per_split = sps;
-      corr_f6_kernel<<<dim3((unsigned)tail, (unsigned)split), dim3(256), 0, s>>>(J, (int)head);
-      CHX_HIP(hipGetLastError());
-    }
+    J.mode = kmode;
+    J.stages_per_split = sps;
+    const int64_t grid = head + tail * split;
+    if (grid > 0x7FFFFFFF) return CHX_ERR_SHAPE;
+    corr_f6_kernel<<<dim3((unsigned)grid), dim3(256), 0, s>>>(J, (int)head, (int)tail, tmode);
+    CHX_HIP(hipGetLastError());
     if (p->prof) CHX_HIP(hipEventRecord(ev[2], s));
   }
   return CHX_SUCCESS;
