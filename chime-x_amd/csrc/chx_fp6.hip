@@ -93,23 +93,28 @@ __device__ __forceinline__ void transpose4(uint32_t r0, uint32_t r1, uint32_t r2
 }
 
 // Pass 1.  Workgroup = 256 element quads (1024 elements) x one 32-sample
-// half of a 64-sample block of one channel.  Thread: 32 word loads (one per
-// sample row; 256 contiguous bytes per wave and row), 8 byte transposes,
-// then per plane: codes, bit packing, LDS staging, and full-run global
-// stores (1 KB of P0 or 512 B of P1 per wave instruction).
-// KB is the destination layout's block count; (kb, h) the destination
-// block and half written; source sample of its first row = 64 (kb -
-// kb_src0) + 32 h.
+// half of a 64-sample block of one channel; an array of Np < 1024 elements
+// takes nh = floor(1024 / Np) consecutive halves instead, so all threads
+// work.  Thread: 32 word loads (one per sample row; 256 contiguous bytes per
+// wave and row), 8 byte transposes, then per plane: codes, bit packing, LDS
+// staging, and full-run global stores (1 KB of P0 or 512 B of P1 per wave
+// instruction; the halves of one unit are consecutive destination rows).
+// Source half hk (relative to t0) lands at destination half 2 kb_dst0 + hk
+// of a layout with KB_dst blocks; the unit covers halves hk0 .. hk0+nh-1.
+template <bool MULTI>  // MULTI: Np < 1024, several halves per unit
 __device__ __forceinline__ void unpack_f6_unit(const uint8_t *__restrict__ in,
                                                uint8_t *__restrict__ p0,
                                                uint8_t *__restrict__ p1, int N, int F, int Np,
                                                int64_t row_bytes, int64_t t0, int64_t t_valid,
-                                               int KB, int qb, int kb, int kb_src0, int h, int f,
-                                               v4u *s0, v2u *s1) {
+                                               int KB_dst, int kb_dst0, int qb, int hk0, int nh,
+                                               int f, v4u *s0, v2u *s1) {
   const int tid = threadIdx.x;
-  const int q = qb * 256 + tid;  // element quad
-  const bool evalid = 4 * q < N;
-  const int64_t ts = (int64_t)kK * (kb - kb_src0) + 32 * h;  // first source sample of the half
+  const int epq = MULTI ? Np / 4 : 256;   // element quads per half in this unit
+  const int sub = MULTI ? tid / epq : 0;  // which half of the unit
+  const bool active = !MULTI || sub < nh;
+  const int q = MULTI ? tid % epq : qb * 256 + tid;  // element quad
+  const bool evalid = active && 4 * q < N;
+  const int64_t ts = 32 * (int64_t)(hk0 + sub);  // first source sample of the half
   uint32_t c[4][8];  // [element][4-sample group]
   {
     const uint8_t *src = in + (t0 + ts) * row_bytes + (int64_t)f * N + 4 * q;
@@ -125,25 +130,29 @@ __device__ __forceinline__ void unpack_f6_unit(const uint8_t *__restrict__ in,
                  c[3][m]);
   }
   const int e_base = qb * 1024;
-  const int n_el = Np - e_base < 1024 ? Np - e_base : 1024;
+  const int n_el = MULTI ? nh * Np : (Np - e_base < 1024 ? Np - e_base : 1024);
+  const int el0 = MULTI ? sub * Np + 4 * q : 4 * tid;  // this thread's entries
 #pragma unroll
   for (int pl = 0; pl < 2; pl++) {
+    if (active) {
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      uint32_t code[8];
+      for (int e = 0; e < 4; e++) {
+        uint32_t code[8];
 #pragma unroll
-      for (int m = 0; m < 8; m++)
-        code[m] = e3m2_codes(pl == 0 ? (c[e][m] >> 4) & 0x0F0F0F0Fu : c[e][m] & 0x0F0F0F0Fu);
-      const int el = 4 * tid + e;
-      const int slot = el + (el >> 4);  // one pad entry per 16
-      v4u a;
-      v2u b;
-      pack32(code, a, b);
-      s0[slot] = a;
-      s1[slot] = b;
+        for (int m = 0; m < 8; m++)
+          code[m] = e3m2_codes(pl == 0 ? (c[e][m] >> 4) & 0x0F0F0F0Fu : c[e][m] & 0x0F0F0F0Fu);
+        const int el = el0 + e;
+        const int slot = el + (el >> 4);  // one pad entry per 16
+        v4u a;
+        v2u b;
+        pack32(code, a, b);
+        s0[slot] = a;
+        s1[slot] = b;
+      }
     }
     __syncthreads();
-    const int64_t row = (((int64_t)pl * F + f) * KB + kb) * 2 + h;  // (plane, channel, block, half)
+    // (plane, channel, destination half) of the unit's first half
+    const int64_t row = ((int64_t)pl * F + f) * 2 * KB_dst + 2 * (int64_t)kb_dst0 + hk0;
     uint8_t *d0 = p0 + (row * Np + e_base) * 16;
     uint8_t *d1 = p1 + (row * Np + e_base) * 8;
 #pragma unroll
@@ -159,25 +168,28 @@ __device__ __forceinline__ void unpack_f6_unit(const uint8_t *__restrict__ in,
   }
 }
 
-// Pass 1.  Workgroup = 256 element quads (1024 elements) x one 32-sample
-// half of a 64-sample block of one channel.  Thread: 32 word loads (one per
-// sample row; 256 contiguous bytes per wave and row), 8 byte transposes,
-// then per plane: codes, bit packing, LDS staging, and full-run global
-// stores (1 KB of P0 or 512 B of P1 per wave instruction).  Grid: x =
-// channel, y = element group, z = (block, half), so that workgroups running
+// Pass 1 over one plan chunk.  Grid: x = channel, y = element group, z =
+// unit of halves (fp6_halves_per_unit(Np) each), so that workgroups running
 // together read whole input rows.
 __global__ __launch_bounds__(256) void unpack_f6_kernel(
     const uint8_t *__restrict__ in, uint8_t *__restrict__ p0, uint8_t *__restrict__ p1, int N,
     int F, int Np, int64_t row_bytes, int64_t t0, int64_t t_valid, int KB) {
   __shared__ __attribute__((aligned(16))) v4u s0[1024 + 64];  // [element] P0, padded
   __shared__ __attribute__((aligned(16))) v2u s1[1024 + 64];  // [element] P1, padded
-  unpack_f6_unit(in, p0, p1, N, F, Np, row_bytes, t0, t_valid, KB, blockIdx.y, blockIdx.z >> 1,
-                 0, blockIdx.z & 1, blockIdx.x, s0, s1);
+  const int g = fp6_halves_per_unit(Np);
+  const int hk0 = blockIdx.z * g;
+  const int nh = 2 * KB - hk0 < g ? 2 * KB - hk0 : g;
+  if (Np >= 1024)
+    unpack_f6_unit<false>(in, p0, p1, N, F, Np, row_bytes, t0, t_valid, KB, 0, blockIdx.y, hk0,
+                          1, blockIdx.x, s0, s1);
+  else
+    unpack_f6_unit<true>(in, p0, p1, N, F, Np, row_bytes, t0, t_valid, KB, 0, 0, hk0, nh,
+                         blockIdx.x, s0, s1);
 }
 
 // Batch: one launch unpacks the chunks of many jobs; workgroup g belongs to
 // the job j with jobs[j].unit0 <= g < jobs[j+1].unit0 (binary search); its
-// unit within the job is ((block, half), element group, channel), channel
+// unit within the job is (unit of halves, element group, channel), channel
 // fastest.
 __global__ __launch_bounds__(256) void unpack_f6_batch_kernel(const UnpackJob *__restrict__ jobs,
                                                               int njobs) {
@@ -192,12 +204,19 @@ __global__ __launch_bounds__(256) void unpack_f6_batch_kernel(const UnpackJob *_
   const UnpackJob &J = jobs[lo];
   const int64_t u = g - J.unit0;
   const int nq = (J.Np + 1023) / 1024;
-  if (u >= (int64_t)nq * 2 * J.KB * J.F) return;  // grid padding (uniform per workgroup)
+  const int hpu = fp6_halves_per_unit(J.Np);
+  const int nhu = (2 * J.KB + hpu - 1) / hpu;  // units of halves per element group
+  if (u >= (int64_t)nq * nhu * J.F) return;  // grid padding (uniform per workgroup)
   const int f = (int)(u % J.F);  // channel fastest: neighbours read whole rows
   const int qb = (int)((u / J.F) % nq);
-  const int hk = (int)(u / ((int64_t)J.F * nq));  // (block, half) within the job
-  unpack_f6_unit(J.in, J.planes, J.planes1, J.N, J.F, J.Np, J.row_bytes, J.t0, J.t_valid,
-                 J.KB_dst, qb, J.kb_dst0 + (hk >> 1), J.kb_dst0, hk & 1, f, s0, s1);
+  const int hk0 = (int)(u / ((int64_t)J.F * nq)) * hpu;  // first half within the job
+  const int nh = 2 * J.KB - hk0 < hpu ? 2 * J.KB - hk0 : hpu;
+  if (J.Np >= 1024)
+    unpack_f6_unit<false>(J.in, J.planes, J.planes1, J.N, J.F, J.Np, J.row_bytes, J.t0,
+                          J.t_valid, J.KB_dst, J.kb_dst0, qb, hk0, 1, f, s0, s1);
+  else
+    unpack_f6_unit<true>(J.in, J.planes, J.planes1, J.N, J.F, J.Np, J.row_bytes, J.t0,
+                         J.t_valid, J.KB_dst, J.kb_dst0, 0, hk0, nh, f, s0, s1);
 }
 
 // Flush one block's FP32 accumulators (exact integers) into int32
@@ -598,7 +617,9 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
       p->prof_ev->insert(p->prof_ev->end(), ev, ev + 3);
       CHX_HIP(hipEventRecord(ev[0], s));
     }
-    unpack_f6_kernel<<<dim3((unsigned)F, (unsigned)((Np + 1023) / 1024), (unsigned)(2 * KB)),
+    unpack_f6_kernel<<<dim3((unsigned)F, (unsigned)((Np + 1023) / 1024),
+                            (unsigned)((2 * KB + fp6_halves_per_unit(Np) - 1) /
+                                       fp6_halves_per_unit(Np))),
                        dim3(256), 0, s>>>(in, p0, p1, N, F, Np, row_bytes, t0, tv, KB);
     CHX_HIP(hipGetLastError());
     if (p->prof) CHX_HIP(hipEventRecord(ev[1], s));

@@ -125,7 +125,8 @@ int launch_unpack_planes(const uint8_t *in, uint8_t *planes, int N, int F, int N
                          hipStream_t s);
 // One job's chunk for the batched unpack, starting at global workgroup
 // unit0 (increasing over jobs): units = ceil(Np/1024) * KB * F (int8) or
-// ceil(Np/1024) * 2 * KB * F (fp6: one workgroup per 32-sample half).  Its
+// ceil(Np/1024) * ceil(2 KB / fp6_halves_per_unit(Np)) * F (fp6: one
+// workgroup per 32-sample half, or per several for arrays under 1024).  Its
 // KB blocks land at blocks [kb_dst0, kb_dst0 + KB) of planes laid out with
 // KB_dst blocks (format layouts above).  t_valid = 0 writes zeros and reads
 // nothing.
@@ -135,8 +136,14 @@ struct UnpackJob {
   int64_t row_bytes, t0, t_valid, unit0;
   int N, F, Np, KB, KB_dst, kb_dst0;
 };
+// 32-sample halves per fp6 unpack workgroup: arrays under 1024 elements
+// take several, so that all 1024 element slots work
+__host__ __device__ inline int fp6_halves_per_unit(int Np) { return Np >= 1024 ? 1 : 1024 / Np; }
 inline int64_t unpack_units(int fmt, const UnpackJob &u) {
-  return (int64_t)((u.Np + 1023) / 1024) * u.KB * u.F * (fmt == kPlanesFp6 ? 2 : 1);
+  const int64_t groups = (u.Np + 1023) / 1024;
+  if (fmt != kPlanesFp6) return groups * u.KB * u.F;
+  const int hpu = fp6_halves_per_unit(u.Np);
+  return groups * ((2 * (int64_t)u.KB + hpu - 1) / hpu) * u.F;
 }
 int launch_unpack_batch(int fmt, const UnpackJob *d_jobs, int njobs, int64_t units,
                         hipStream_t s);
