@@ -74,6 +74,63 @@ int add_job_items(int fmt, int gj, int F, int ntiles, int nst, int max_st,
   return pieces;
 }
 
+int cut_partial_rounds(int fmt, std::vector<GemmItem> &items, std::vector<int> &qbound,
+                       std::vector<GemmJob> &gjobs, int nwg) {
+  if (qbound.size() != 9 || nwg < 8) return 0;
+  const int wq = nwg / 8;  // workgroups that start on each queue
+  const int ring = format_ring(fmt);
+  std::vector<int> tail_job(gjobs.size(), -1);
+  std::vector<GemmItem> out;
+  std::vector<int> qb2(1, 0);
+  int added = 0;
+  for (int q = 0; q < 8; q++) {
+    const int b0 = qbound[q], b1 = qbound[q + 1];
+    const int n = b1 - b0, r = wq > 0 ? n % wq : 0;
+    bool equal = n > 0;
+    for (int k = b0; k < b1 && equal; k++)
+      equal = items[k].nstages == items[b0].nstages && gjobs[items[k].job].mode != 2;
+    int best_p = 1;
+    if (equal && n > wq && r > 0) {
+      const int len = items[b0].nstages;
+      double best = 1.0;  // the partial round's length in whole items
+      for (int p = 2; p <= 8 && len / p >= 16; p++) {
+        const double rounds = (double)((r * p + wq - 1) / wq) / p;
+        if (rounds < best - 1e-9) {
+          best = rounds;
+          best_p = p;
+        }
+      }
+    }
+    if (best_p == 1) {
+      out.insert(out.end(), items.begin() + b0, items.begin() + b1);
+    } else {
+      // the queue's last r items become pieces, owned by copies of their
+      // jobs that add atomically, after the whole items
+      out.insert(out.end(), items.begin() + b0, items.begin() + (b1 - r));
+      const int len = items[b0].nstages;
+      const int per = ((len + best_p - 1) / best_p + ring - 1) / ring * ring;
+      for (int k = b1 - r; k < b1; k++) {
+        const GemmItem &w = items[k];
+        if (tail_job[w.job] < 0) {
+          GemmJob j = gjobs[w.job];
+          j.mode = 2;
+          tail_job[w.job] = (int)gjobs.size();
+          gjobs.push_back(j);
+          added++;
+        }
+        for (int s0 = 0; s0 < w.nstages; s0 += per) {
+          const int ns = per < w.nstages - s0 ? per : w.nstages - s0;
+          out.push_back({tail_job[w.job], w.f, w.tile, w.stage0 + s0, ns});
+        }
+      }
+    }
+    qb2.push_back((int)out.size());
+  }
+  items.swap(out);
+  qbound.swap(qb2);
+  return added;
+}
+
 void spread_over_queues(std::vector<ItemGroup> &groups, std::vector<GemmItem> &items,
                         std::vector<int> &qbound) {
   // Longest-processing-time first: heaviest group to the lightest queue.

@@ -164,6 +164,16 @@ int add_job_items(int fmt, int gj, int F, int ntiles, int nst, int max_st,
 // bounds to `qbound`.
 void spread_over_queues(std::vector<ItemGroup> &groups, std::vector<GemmItem> &items,
                         std::vector<int> &qbound);
+// After spread_over_queues with one round of 8 queue bounds: in each queue
+// whose n items are equal and whole, with n % (nwg / 8) = r > 0 and n >
+// nwg / 8, the last r items are cut into time pieces (the count that
+// minimises the partial round's length, >= 16 stages each) owned by copies
+// of their jobs with mode 2 (atomic add), so that the last round keeps every
+// workgroup of the queue busy.  Appends the job copies to gjobs; returns how
+// many.  The caller's outputs must take atomic adds (zeroed, or
+// accumulating).
+int cut_partial_rounds(int fmt, std::vector<GemmItem> &items, std::vector<int> &qbound,
+                       std::vector<GemmJob> &gjobs, int nwg);
 // Invariants the kernels rely on, checked on the host before a launch:
 // every plane region inside the workspace [base, base + work_bytes), every
 // item inside its job's chunk (whole rings), `rounds` sets of 9 queue bounds

@@ -186,6 +186,9 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
   std::vector<GemmItem> items;
   std::vector<int> qbound;
   spread_over_queues(groups, items, qbound);
+  // accumulators take atomic adds: a queue's last partial round may be cut
+  // in time
+  if (S->cfg.max_item_stages == 0) cut_partial_rounds(S->fmt, items, qbound, gjobs, S->workgroups);
   int rc = check_launch_tables(S->fmt, ujobs, gjobs, items, qbound, 1, S->d_work,
                                (size_t)S->cfg.workspace_bytes);
   if (rc) return rc;

@@ -89,6 +89,9 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=64)
     ap.add_argument("--loads", default="0.5,1,2", help="arrival rates as multiples of sustainable")
     ap.add_argument("--host-pool", type=int, default=32, help="pinned frames cycled by host runs")
+    ap.add_argument("--max-item-stages", type=int, default=0,
+                    help="cut GEMM work items into pieces of this many 64-sample stages (0 = whole)")
+    ap.add_argument("--modes", default="device_resident,host_pcie")
     ap.add_argument("--out", default="")
     a = ap.parse_args()
 
@@ -112,9 +115,12 @@ def main() -> int:
     loads = [float(x) for x in a.loads.split(",")]
     res = dict(workload=f"streaming: N={N}, F={F}, frames of {T} samples, {n} frames "
                         f"({n * T} samples), 5% of elements held at (7,-8)", runs={})
-    with chx.Serve(frame_slots=a.slots, slot_bytes=fb) as sv:
+    with chx.Serve(frame_slots=a.slots, slot_bytes=fb,
+                   max_item_stages=a.max_item_stages) as sv:
         for mode, frames, device in (("device_resident", dev_frames, True),
                                      ("host_pcie", host_frames, False)):
+            if mode not in a.modes.split(","):
+                continue
             sid = sv.open(N, F, T)
             sat = saturated(sv, sid, frames, device)
             rate = sat["achieved_rate"]
