@@ -308,17 +308,27 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
     const int i = lane & 31;
     voff1[pp] = ((i >> 4) * Np + group_elem(g) + 2 * (i & 15)) * 8;
   }
+  // Groups past the array (the right half of a row's last tile when the
+  // tile count is odd) load through an empty descriptor: every offset is
+  // out of range, so nothing is fetched (their products are never stored).
+  const __amdgpu_buffer_rsrc_t rz = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)J.planes, (short)0, 0, 0x00020000);
+  const bool past = (ct + 1) * kTile >= Np;  // groups 8-11 lie past the array
   auto issue_stage = [&](int st, int slot) {
     const int kb = st_begin + st;
 #pragma unroll
     for (int gg = 0; gg < 6; gg++) {
-      lds_void *dst = (lds_void *)&lds[slot][(lp * kGroups + 6 * gh + gg) * 1024];
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(r0, dst, 16, voff0[gg], kb * (int)p0_blk, 0, 0);
+      const int g = 6 * gh + gg;
+      lds_void *dst = (lds_void *)&lds[slot][(lp * kGroups + g) * 1024];
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(g >= 8 && past ? rz : r0, dst, 16, voff0[gg],
+                                               kb * (int)p0_blk, 0, 0);
     }
 #pragma unroll
     for (int pp = 0; pp < 3; pp++) {
-      lds_void *dst = (lds_void *)&lds[slot][kP0Bytes + (lp * kGroups + 6 * gh + 2 * pp) * 512];
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(r1, dst, 16, voff1[pp], kb * (int)p1_blk, 0, 0);
+      const int g = 6 * gh + 2 * pp;
+      lds_void *dst = (lds_void *)&lds[slot][kP0Bytes + (lp * kGroups + g) * 512];
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(g >= 8 && past ? rz : r1, dst, 16, voff1[pp],
+                                               kb * (int)p1_blk, 0, 0);
     }
   };
 
