@@ -171,8 +171,9 @@ typedef struct chx_job_desc {
 typedef struct chx_batch_tuning {
   int32_t max_item_stages;  /* time extent of one work item, in 64-sample
                                stages (rounded up to a multiple of the
-                               method's ring: 4 int8, 6 FP6);
-                               0 = a whole chunk per item */
+                               method's ring, 4 stages); 0 = a whole chunk
+                               per item (FP6: at most 2048 stages, longer
+                               chunks are cut) */
   int32_t workgroups;       /* persistent workgroups (multiple of 8);
                                0 = one per compute unit */
   int64_t workspace_bytes;  /* HBM budget of one round's planes;
@@ -240,7 +241,9 @@ typedef struct chx_serve_config {
                                max_batch_frames * slot_bytes */
   int64_t hbm_budget_bytes; /* slots + workspace + accumulators; 0 = no cap */
   int32_t max_batch_frames; /* frames per launch at most; 0 = frame_slots */
-  int32_t max_item_stages;  /* as in chx_batch_tuning; 0 = whole */
+  int32_t max_item_stages;  /* as in chx_batch_tuning; 0 = whole items, and
+                               when a queue's items are equal, its last
+                               partial round cut in time */
   int32_t method;           /* CHX_METHOD_INT8_MFMA or CHX_METHOD_FP6_MFMA;
                                0 = FP6 */
   int32_t reserved;         /* 0 */
