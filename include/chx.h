@@ -105,16 +105,16 @@ int chx_preseed(const chx_plan *plan, const uint32_t *d_sums, int32_t *d_vis,
 /* corr (pairwise_correlator{,_UT}.cl:22-207,
  * packed_correlator_overflow_protected_to_2272_iter{,_UT}.cl): adds the
  * method's products into d_vis.  Methods 0/1 add the raw nibble-space
- * products (d_vis must hold the preseed); method 2 adds the finished signed
- * visibilities.  No lock buffer: blocks are owned, split time ranges add
+ * products (d_vis must hold the preseed); methods 2 and 3 add the finished
+ * signed visibilities.  No lock buffer: blocks are owned, split time ranges add
  * with atomics. */
 int chx_corr(const chx_plan *plan, const uint8_t *d_input, int32_t *d_vis,
              void *stream);
 
 /* The whole per-iteration pipeline (main_wrapper.c:691-778): assigns the
  * final visibilities of d_input to d_vis.  Methods 0/1: zero sums ->
- * offset accumulate -> preseed -> corr, with plan-owned sums; method 2: one
- * matrix-core pass. */
+ * offset accumulate -> preseed -> corr, with plan-owned sums; methods 2 and
+ * 3: one matrix-core pass per plane chunk. */
 int chx_correlate(chx_plan *plan, const uint8_t *d_input, int32_t *d_vis,
                   void *stream);
 
@@ -123,7 +123,7 @@ int chx_correlate(chx_plan *plan, const uint8_t *d_input, int32_t *d_vis,
 int chx_correlate_host(chx_plan *plan, const uint8_t *h_input, int32_t *h_vis);
 
 /* Kernel timing (no reference counterpart; for roofline measurements):
- * while enabled, every int8 matrix-core chunk the plan launches records HIP
+ * while enabled, every matrix-core chunk (int8 or FP6) the plan launches records HIP
  * events on its stream before the unpack kernel, between it and the GEMM
  * kernel, and after the GEMM kernel.  chx_plan_kernel_times waits for the
  * recorded launches, returns the summed unpack and GEMM durations and the
