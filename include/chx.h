@@ -159,7 +159,7 @@ int chx_stream_synchronize(void *stream);
 typedef struct chx_job_desc {
   int32_t num_elements;              /* N: multiple of 32, >= 32 */
   int32_t num_freq;                  /* F >= 1 */
-  int64_t time_steps;                /* T >= 1 (no time_accum: method 2) */
+  int64_t time_steps;                /* T >= 1 (no time_accum: matrix cores) */
   int32_t upper_triangle_convention; /* 1 standard, 0 conjugate */
   int32_t reserved;                  /* must be 0 */
   const uint8_t *d_input;            /* [T][F][N] bytes on the batch's device */
