@@ -346,14 +346,31 @@ __device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti, int &tj
 
 // One plan: grid.x = channels x upper-triangle tiles (XCD-remapped), grid.y
 // = time splits.
-__global__ __launch_bounds__(256, 1) void corr_mfma_kernel(GemmJob J) {
+// One plan chunk: items = channels x tiles.  Workgroups 0..head-1 take
+// items 0..head-1 whole (all stages, mode J.mode); the rest take the tail
+// items head.. cut into time pieces of J.stages_per_split stages (the last
+// piece the rest), mode tmode.  Each range is XCD-remapped.
+__global__ __launch_bounds__(256, 1) void corr_mfma_kernel(GemmJob J, int nst, int head,
+                                                           int tail, int tmode) {
   __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kStageBytes];
-  const int64_t item = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
   const int64_t per_f = (int64_t)J.ntiles * (J.ntiles + 1) / 2;
+  int64_t item;
+  int s0, ns;
+  if ((int)blockIdx.x < head) {
+    item = xcd_remap(blockIdx.x, head);
+    s0 = 0;
+    ns = nst;
+  } else {
+    const int64_t pieces = (int64_t)gridDim.x - head;
+    const int64_t r = xcd_remap((int64_t)blockIdx.x - head, pieces);
+    item = head + r % tail;
+    s0 = (int)(r / tail) * J.stages_per_split;
+    ns = nst - s0 < J.stages_per_split ? nst - s0 : J.stages_per_split;
+    J.mode = tmode;
+  }
   int ti, tj;
   tile_coords((int)(item % per_f), J.ntiles, ti, tj);
-  gemm_tile(J, (int)(item / per_f), ti, tj, blockIdx.y * J.stages_per_split,
-            J.stages_per_split, lds);
+  gemm_tile(J, (int)(item / per_f), ti, tj, s0, ns, lds);
 }
 
 // Batch: persistent workgroups take work items (job, channel, tile, stage
@@ -424,6 +441,9 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
   const int ntiles = p->ntiles, Np = ntiles * kTile;
   const int64_t items = (int64_t)F * ntiles * (ntiles + 1) / 2;
   if (items > 0x7FFFFFFF || !p->d_work) return CHX_ERR_SHAPE;
+  int cus = 0;
+  CHX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->cfg.device));
+  if (cus < 1) cus = 1;
   const int64_t tc = chunk_samples(p);
   const int64_t row_bytes = (int64_t)F * N;
   for (int64_t t0 = 0; t0 < T; t0 += tc) {
@@ -431,6 +451,32 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
     // stages, padded to a multiple of the ring (the padding samples are 0)
     const int nst_chunk = (int)((tv + kStage * kRing - 1) / (kStage * kRing)) * kRing;
     const int KB = nst_chunk * (kStage / 16);
+    // Launch shape (one dispatch): whole rounds of one workgroup per CU take
+    // whole items with plain stores; the items of a last partial round are
+    // cut in time into the fewest pieces (whole rings, >= 16 stages) that
+    // minimise its length, adding atomically into zeroed output.
+    const int64_t head = items / cus * cus, tail = items - head;
+    int split = 1;
+    if (tail > 0) {
+      double best = (double)((tail + cus - 1) / cus);
+      for (int sp = 2; sp <= 64 && nst_chunk / sp >= 16; sp++) {
+        const double r = (double)((tail * sp + cus - 1) / cus) / sp;
+        if (r < best - 1e-9) {
+          best = r;
+          split = sp;
+        }
+      }
+    }
+    const int sps = ((nst_chunk + split - 1) / split + kRing - 1) / kRing * kRing;
+    split = (nst_chunk + sps - 1) / sps;
+    const int kmode = (t0 == 0) ? mode : 1;
+    const int tmode = split > 1 ? 2 : kmode;
+    if (tail > 0 && split > 1 && kmode == 0) {
+      const int64_t f0 = head / ((int64_t)ntiles * (ntiles + 1) / 2);
+      int rc = launch_fill_u32((uint32_t *)vis + f0 * p->NB * 2048, 0u,
+                               (int64_t)(F - f0) * p->NB * 2048, s);
+      if (rc) return rc;
+    }
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     if (p->prof) {
       for (hipEvent_t &e : ev) CHX_HIP(hipEventCreate(&e));
@@ -440,20 +486,6 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
     int rc0 = launch_unpack_planes(in, p->d_work, N, F, Np, row_bytes, t0, tv, KB, s);
     if (rc0) return rc0;
     if (p->prof) CHX_HIP(hipEventRecord(ev[1], s));
-    // Split time only when the tiles alone cannot fill the chip twice over,
-    // keeping >= 16 stages per split.
-    int split = (int)((512 + items - 1) / items);
-    if (split > nst_chunk / 16) split = nst_chunk / 16;
-    if (split < 1) split = 1;
-    while ((nst_chunk / kRing) % split) split--;  // splits hold whole rings
-    int kmode = (t0 == 0) ? mode : 1;
-    if (split > 1) {
-      if (kmode == 0) {
-        int rc = launch_fill_u32((uint32_t *)vis, 0u, (int64_t)F * p->NB * 2048, s);
-        if (rc) return rc;
-      }
-      kmode = 2;
-    }
     GemmJob J;
     J.planes = p->d_work;
     J.planes1 = nullptr;
@@ -466,8 +498,11 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
     J.nb = p->nb;
     J.U = U;
     J.mode = kmode;
-    J.stages_per_split = nst_chunk / split;
-    corr_mfma_kernel<<<dim3((unsigned)items, (unsigned)split), dim3(256), 0, s>>>(J);
+    J.stages_per_split = sps;
+    const int64_t grid = head + tail * split;
+    if (grid > 0x7FFFFFFF) return CHX_ERR_SHAPE;
+    corr_mfma_kernel<<<dim3((unsigned)grid), dim3(256), 0, s>>>(J, nst_chunk, (int)head,
+                                                                  (int)tail, tmode);
     CHX_HIP(hipGetLastError());
     if (p->prof) CHX_HIP(hipEventRecord(ev[2], s));
   }
