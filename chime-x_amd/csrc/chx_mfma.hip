@@ -15,16 +15,17 @@
 //  1. unpack_planes_kernel (HBM-bound): [time][freq][element] bytes ->
 //     MFMA-ready planes [plane][freq][16-sample block][element][16 B].  A
 //     byte with nibbles (R, I) is XORed with 0x88 so each nibble holds its
-//     value's 4-bit two's complement; masked into the high half of an int8
-//     it is 16*value exactly.  Elements past N and samples past T are 0.
+//     value's 4-bit two's complement; placed in bits 6..3 of an int8 with
+//     its sign extended into bit 7 it is 8*value exactly.  Elements past N
+//     and samples past T are 0.
 //  2. corr_mfma_kernel (matrix-core-bound): a workgroup = 4 wavefronts = a
 //     128x128-element tile of one channel (16 reference blocks), each wave a
 //     64x64 quarter (2x2 blocks x 3 int32 accumulators), fed by buffer loads
 //     that go straight to LDS (no VGPR staging, no VALU) through a ring of
 //     4 stages of 64 samples; waves only read operands and issue MFMAs.
-// Every product is 256x the true one; accumulators hold at most kChunk
-// samples (|RE| <= 256*128*kChunk = 2^30) and are divided by 256 (exact)
-// when flushed to the int32 output.
+// Every product is 64x the true one; each accumulator holds one product
+// term over at most kChunk samples (|sum| <= 64*64*kChunk = 2^29) and is
+// divided by 64 (exact) when flushed to the int32 output.
 #include "chx_internal.h"
 
 namespace chx {
@@ -39,7 +40,7 @@ typedef __attribute__((address_space(3))) void lds_void;
 constexpr int kTile = kMfmaTile;   // elements per tile side
 constexpr int kStage = kMfmaStage; // samples per LDS stage (2 MFMA k-steps)
 constexpr int kRing = kMfmaRing;    // LDS stages in flight
-constexpr int kChunk = 32768;  // samples accumulated before a flush
+constexpr int kChunk = 131072;  // samples accumulated before a flush
 // LDS stage layout: [plane 2][row group 8][16-sample block 4][32 rows][16 B].
 // A 32-row, 32-sample operand is 1 KB contiguous: lanes 0-31 read block
 // 2k, lanes 32-63 block 2k+1, 16 lanes = 256 contiguous bytes (no bank
@@ -63,11 +64,11 @@ __device__ __forceinline__ void transpose4(uint32_t r0, uint32_t r1, uint32_t r2
   c3 = __builtin_amdgcn_perm(t23h, t01h, 0x07060302u);
 }
 
-// Real and imaginary planes of four packed samples, each value times 16.
+// Real and imaginary planes of four packed samples, each value times 8.
 __device__ __forceinline__ void planes(uint32_t w, uint32_t &re, uint32_t &im) {
-  const uint32_t t = w ^ 0x88888888u;
-  re = t & 0xF0F0F0F0u;
-  im = (t << 4) & 0xF0F0F0F0u;
+  const uint32_t t = w ^ 0x88888888u;  // per byte: R two's complement in 7..4, I in 3..0
+  re = ((t >> 1) & 0x78787878u) | (t & 0x80808080u);         // 8 R, per byte
+  im = ((t << 3) & 0x78787878u) | ((t << 4) & 0x80808080u);  // 8 I, per byte
 }
 
 // Maps the dispatch-order workgroup id so that the workgroups resident on
@@ -92,8 +93,8 @@ __device__ __forceinline__ void flush_block(int32_t *blk, const v16i &rea, const
   const int voff = ((lane >> 5) * 4 * 32 + (lane & 31)) * 8;
 #pragma unroll
   for (int j = 0; j < 16; j++) {
-    const int32_t vr = (rea[j] >> 8) + (reb[j] >> 8);
-    const int32_t vi = U ? (ix[j] - iy[j]) >> 8 : (iy[j] - ix[j]) >> 8;
+    const int32_t vr = (rea[j] >> 6) + (reb[j] >> 6);
+    const int32_t vi = U ? (ix[j] >> 6) - (iy[j] >> 6) : (iy[j] >> 6) - (ix[j] >> 6);
     const int soff = (8 * (j >> 2) + (j & 3)) * 32 * 8;  // row m = 8(j/4) + 4(l/32) + j%4
     if (mode == 0) {
       v2i v = {vr, vi};
