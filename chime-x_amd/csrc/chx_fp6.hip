@@ -9,7 +9,7 @@
 //   RE = Ar.Br^T + Ai.Bi^T,  IM = Ar.Bi^T + (-Ai).Br^T,  im = U ? -IM : IM,
 // where -Ai is Ai with the sign bit of every code flipped (exact).  Every
 // product is an integer of magnitude <= 64, so a sample adds at most 128 to
-// an accumulator; a work item covers at most kFp6MaxStages = 2048 stages of
+// an accumulator; a work item covers at most kMaxItemStages = 2048 stages of
 // 64 samples, so every partial sum is an integer of magnitude <= 2^24 and
 // exact in FP32: the int32 visibilities are bit-identical to the int8
 // path's.
@@ -44,7 +44,7 @@ typedef __attribute__((address_space(3))) void lds_void;
 constexpr int kTile = kMfmaTile;     // 128: tile rows; a tile has 2 * kTile columns
 constexpr int kK = 64;               // samples per k-step (= per stage)
 constexpr int kRing = kFp6Ring;      // LDS stages in the ring
-constexpr int kMaxStages = kFp6MaxStages;  // stages of one work item
+constexpr int kMaxStages = kMaxItemStages;  // stages of one work item
 constexpr int kGroups = 12;          // 32-element groups per stage: 4 of rows, 8 of columns
 constexpr int kP0Bytes = 2 * kGroups * 1024;  // [plane 2][group 12][64 lanes][16 B]
 constexpr int kP1Bytes = 2 * kGroups * 512;   // [plane 2][group 12][64 lanes][8 B]
@@ -474,7 +474,7 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
 __global__ __launch_bounds__(256, 1) void corr_f6_kernel(GemmJob J, int head, int tail,
                                                          int tmode) {
   __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kSlotBytes];
-  const int64_t per_f = fp6_tiles(J.ntiles);
+  const int64_t per_f = wide_tiles(J.ntiles);
   int64_t item;
   int s0, ns;
   if ((int)blockIdx.x < head) {
@@ -576,7 +576,7 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
   const int N = p->cfg.num_elements, F = p->cfg.num_freq, U = p->cfg.upper_triangle_convention;
   const int64_t T = p->cfg.time_steps;
   const int ntiles = p->ntiles, Np = ntiles * kTile;
-  const int64_t items = (int64_t)F * fp6_tiles(ntiles);
+  const int64_t items = (int64_t)F * wide_tiles(ntiles);
   if (items > 0x7FFFFFFF || !p->d_work) return CHX_ERR_SHAPE;
   int cus = 0;
   CHX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->cfg.device));
@@ -616,7 +616,7 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
     const int tmode = split > 1 ? 2 : kmode;  // the tail's pieces add atomically
     if (tail > 0 && split > 1 && kmode == 0) {
       // zero the channels the tail touches (before the head's stores)
-      const int64_t f0 = head / fp6_tiles(ntiles);
+      const int64_t f0 = head / wide_tiles(ntiles);
       int rc = launch_fill_u32((uint32_t *)vis + f0 * p->NB * 2048, 0u,
                                (int64_t)(F - f0) * p->NB * 2048, s);
       if (rc) return rc;

@@ -64,13 +64,16 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis,
                      int mode, hipStream_t s);
 
 // FP6 (E3M2) matrix-core path (chx_fp6.hip): same modes, bit-identical
-// results.  Output tiles are 128 rows x 256 columns: row tile ti holds
-// ceil((ntiles - ti) / 2) tiles, the columns starting at row tile ti.  A
-// work item covers at most kFp6MaxStages stages, so that every FP32 partial
-// sum (at most 128 * 64 per stage) stays <= 2^24, where FP32 is exact.
-constexpr int kFp6Ring = 4;           // 64-sample stages in the LDS ring
-constexpr int kFp6MaxStages = 2048;   // stages of one work item
-__host__ __device__ inline int64_t fp6_tiles(int64_t ntiles) {
+// results.
+constexpr int kFp6Ring = 4;  // 64-sample stages in the LDS ring
+// Both matrix-core GEMMs use output tiles of 128 rows x 256 columns: row
+// tile ti holds ceil((ntiles - ti) / 2) tiles, the columns starting at row
+// tile ti.  A work item covers at most kMaxItemStages 64-sample stages, so
+// that every partial sum stays exact: FP6's FP32 sums (at most 128 * 64 per
+// stage) stay <= 2^24, int8's int32 sums (64x products, at most 8192 * 64
+// per stage) <= 2^30.
+constexpr int kMaxItemStages = 2048;
+__host__ __device__ inline int64_t wide_tiles(int64_t ntiles) {
   const int64_t m = ntiles / 2;
   return ntiles % 2 ? (m + 1) * (m + 1) : m * (m + 1);
 }
@@ -86,12 +89,10 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
 //        64-sample blocks, 1.5 bytes per (element, sample).
 enum PlaneFormat { kPlanesInt8 = 0, kPlanesFp6 = 1 };
 inline int format_ring(int fmt) { return fmt == kPlanesFp6 ? kFp6Ring : kMfmaRing; }
-// output tiles of one channel (int8: 128 x 128 over the upper triangle)
-inline int64_t format_tiles(int fmt, int64_t ntiles) {
-  return fmt == kPlanesFp6 ? fp6_tiles(ntiles) : ntiles * (ntiles + 1) / 2;
-}
-// the most stages one work item may cover (int8: no limit)
-inline int format_max_stages(int fmt) { return fmt == kPlanesFp6 ? kFp6MaxStages : 1 << 30; }
+// output tiles of one channel (both formats: 128 x 256)
+inline int64_t format_tiles(int, int64_t ntiles) { return wide_tiles(ntiles); }
+// the most stages one work item may cover
+inline int format_max_stages(int) { return kMaxItemStages; }
 // stages per work item of an nst-stage chunk cut at max_st (0 = whole)
 inline int format_item_stages(int fmt, int nst, int max_st) {
   int per = (max_st > 0 && max_st < nst) ? max_st : nst;

@@ -3,12 +3,11 @@
 // Each channel's visibilities are a Hermitian rank-T update.  With the
 // samples split into a real plane r = R-8 and an imaginary plane i = I-8
 // (one int8 per value), a 32x32 reference block (rows y, columns x,
-// pairwise_correlator_UT.cl:161-201) is four int8 GEMMs over time:
+// pairwise_correlator_UT.cl:161-201) is two int8 accumulations over time:
 //   RE = Ar.Br^T + Ai.Bi^T      (sum y_r x_r + y_i x_i)
-//   IX = Ai.Br^T                (sum y_i x_r)
-//   IY = Ar.Bi^T                (sum y_r x_i)
-//   im = IX - IY for upper_triangle_convention 1 (cpu_corr_test.c:189),
-//        IY - IX for 0 (cpu_corr_test.c:92).
+//   IM = Ar.Bi^T - Ai.Br^T      (sum y_r x_i - y_i x_r)
+//   im = -IM for upper_triangle_convention 1 (cpu_corr_test.c:189),
+//        IM for 0 (cpu_corr_test.c:92).
 // No offset/preseed correction is needed and there is no time_accum cap.
 //
 // Two passes per chunk of time:
@@ -18,14 +17,16 @@
 //     value's 4-bit two's complement; placed in bits 6..3 of an int8 with
 //     its sign extended into bit 7 it is 8*value exactly.  Elements past N
 //     and samples past T are 0.
-//  2. corr_mfma_kernel (matrix-core-bound): a workgroup = 4 wavefronts = a
-//     128x128-element tile of one channel (16 reference blocks), each wave a
-//     64x64 quarter (2x2 blocks x 3 int32 accumulators), fed by buffer loads
-//     that go straight to LDS (no VGPR staging, no VALU) through a ring of
-//     4 stages of 64 samples; waves only read operands and issue MFMAs.
-// Every product is 64x the true one; each accumulator holds one product
-// term over at most kChunk samples (|sum| <= 64*64*kChunk = 2^29) and is
-// divided by 64 (exact) when flushed to the int32 output.
+//  2. corr_mfma_kernel: a workgroup = 4 wavefronts = a 128-row x 256-column
+//     tile of one channel, each wave 64 x 128 (2x4 blocks x 2 int32
+//     accumulators: RE = Ar.Br + Ai.Bi and IM = Ar.Bi + (-Ai).Br, -Ai
+//     negated per byte in registers), fed by buffer loads that go straight
+//     to LDS through a ring of 4 k-steps of 32 samples (24 KB each), three
+//     in flight.
+// Every product is 64x the true one; a work item covers at most
+// kMaxItemStages 64-sample stages, so every accumulator stays a multiple of
+// 64 of magnitude <= 2^30, divided by 64 (exact) when flushed to the int32
+// output.
 #include "chx_internal.h"
 
 namespace chx {
@@ -37,16 +38,14 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) void lds_void;
 
-constexpr int kTile = kMfmaTile;   // elements per tile side
-constexpr int kStage = kMfmaStage; // samples per LDS stage (2 MFMA k-steps)
-constexpr int kRing = kMfmaRing;    // LDS stages in flight
-constexpr int kChunk = 131072;  // samples accumulated before a flush
-// LDS stage layout: [plane 2][row group 8][16-sample block 4][32 rows][16 B].
-// A 32-row, 32-sample operand is 1 KB contiguous: lanes 0-31 read block
-// 2k, lanes 32-63 block 2k+1, 16 lanes = 256 contiguous bytes (no bank
+constexpr int kTile = kMfmaTile;  // 128: tile rows; a tile has 2 * kTile columns
+constexpr int kRing = 4;           // LDS slots of one 32-sample k-step
+constexpr int kGroups = 12;        // 32-element groups per k-step: 4 of rows, 8 of columns
+// LDS k-step slot: [plane 2][group 12][1 KB operand].  A 32-row, 32-sample
+// operand is 1 KB contiguous: lanes 0-31 read the k-step's even 16-sample
+// block, lanes 32-63 the odd one, 16 lanes = 256 contiguous bytes (no bank
 // conflicts); one 64-lane direct load fills exactly one operand.
-constexpr int kGroupBytes = 4 * 512;
-constexpr int kStageBytes = 2 * 8 * kGroupBytes;  // 32 KB
+constexpr int kSlotBytes = 2 * kGroups * 1024;  // 24 KB
 
 // 4x4 byte transpose: r[s] holds byte e of sample s; c[e] gets byte s of
 // element e.  v_perm_b32 selects bytes of {s0:s1} (selector 0-3 = s1,
@@ -82,10 +81,10 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t id, int64_t total) {
 
 // Flush one block's accumulators.  Lane l owns column n = l%32 and rows
 // m = 8*(j/4) + 4*(l/32) + j%4 of register j (v_mfma_i32_32x32x32_i8 D
-// layout).  mode 0 assigns, 1 adds, 2 adds atomically.
-__device__ __forceinline__ void flush_block(int32_t *blk, const v16i &rea, const v16i &reb,
-                                            const v16i &ix, const v16i &iy, int U, int mode,
-                                            int lane) {
+// layout).  re holds 64 (Ar.Br + Ai.Bi), im 64 (Ar.Bi - Ai.Br).  mode 0
+// assigns, 1 adds, 2 adds atomically.
+__device__ __forceinline__ void flush_block(int32_t *blk, const v16i &re, const v16i &im, int U,
+                                            int mode, int lane) {
   // Buffer ops on a descriptor of the block (a wave-uniform base): one VGPR
   // of lane offset, the register's row as a constant scalar offset.
   const __amdgpu_buffer_rsrc_t rsrc =
@@ -93,8 +92,8 @@ __device__ __forceinline__ void flush_block(int32_t *blk, const v16i &rea, const
   const int voff = ((lane >> 5) * 4 * 32 + (lane & 31)) * 8;
 #pragma unroll
   for (int j = 0; j < 16; j++) {
-    const int32_t vr = (rea[j] >> 6) + (reb[j] >> 6);
-    const int32_t vi = U ? (ix[j] >> 6) - (iy[j] >> 6) : (iy[j] >> 6) - (ix[j] >> 6);
+    const int32_t vr = re[j] >> 6;  // exact: every product is a multiple of 64
+    const int32_t vi = U ? -(im[j] >> 6) : im[j] >> 6;
     const int soff = (8 * (j >> 2) + (j & 3)) * 32 * 8;  // row m = 8(j/4) + 4(l/32) + j%4
     if (mode == 0) {
       v2i v = {vr, vi};
@@ -201,160 +200,203 @@ __global__ __launch_bounds__(256) void unpack_batch_kernel(const UnpackJob *__re
               J.kb_dst0 + kb, J.kb_dst0, f, stage);
 }
 
-// Pass 2, one output tile: the (ti, tj) 128x128-element tile of channel f
-// of one job, over stages [st_begin, st_begin + nst) (nst a multiple of
-// kRing).
-// Shared by the per-plan kernel (one tile per workgroup) and the persistent
-// batch kernel (tiles taken from a work list).  J.mode: 0 assign, 1 add
-// (first flush; later flushes of the tile add), 2 atomic add.
+// Tile tb of one channel: rows [kTile ti, +kTile), columns [kTile ct,
+// +2 kTile), row-major over ti, then ct = ti, ti + 2, ...  Row ti holds
+// ceil((ntiles - ti) / 2) tiles; the last one's second column half lies past
+// the array when ntiles - ti is odd.
+__device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti, int &ct) {
+  int r = 0;
+  while (tb >= (ntiles - r + 1) / 2) {
+    tb -= (ntiles - r + 1) / 2;
+    r++;
+  }
+  ti = r;
+  ct = r + 2 * tb;
+}
+
+// Pass 2, one output tile (rows from kTile ti, columns from kTile ct) of
+// channel f over 64-sample stages [st_begin, st_begin + nst) (nst a multiple
+// of kMfmaRing), run as 32-sample k-steps.  Wave w computes rows 64 (w / 2)..
+// x columns 128 (w % 2).. as 2x4 blocks of two int32 accumulators (256
+// AGPRs):  RE += Ar.Br + Ai.Bi,  IM += Ar.Bi + (-Ai).Br,  where -Ai is
+// negated per byte in registers (exact: every byte is 8 x value, in
+// [-64, 56]).
 __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const int ti,
-                                          const int tj, const int st_begin, const int nst,
-                                          uint8_t (*lds)[kStageBytes]) {
+                                          const int ct, const int st_begin, const int nst,
+                                          uint8_t (*lds)[kSlotBytes]) {
   const int F = J.F, Np = J.Np, KB = J.KB, nb = J.nb, U = J.U, mode = J.mode;
   const int64_t NB = J.NB;
-  const uint8_t *planes = J.planes;
   int32_t *vis = J.vis;
-  // wave index made provably uniform: descriptors and LDS destinations built
-  // from it stay in scalar registers.
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int k0 = 2 * st_begin, nk = 2 * nst;  // k-steps
 
-  // Loads: wave w fills the 8 operands (plane w/2, row groups 4(w&1)..+3,
-  // k-steps 0/1) of each stage.  One descriptor per (plane, channel); the
-  // lane offset picks the element and the 16-sample block parity, the
-  // scalar offset the stage's block.
-  const int lp = wave >> 1;
+  // Loads: wave w fills plane w/2, groups 6(w&1)..+5 of each k-step (one
+  // 1 KB operand each: lanes 0-31 the k-step's even 16-sample block, 32-63
+  // the odd one).  Group g < 4 holds rows kTile ti + 32 g, g >= 4 columns
+  // kTile ct + 32 (g - 4); groups past the array (the right half of a row's
+  // last tile when the tile count is odd) load through an empty descriptor:
+  // every offset is out of range, nothing is fetched, their products are
+  // never stored.
+  const int lp = wave >> 1, gh = wave & 1;
   const int64_t blk_bytes = (int64_t)Np * 16;  // one 16-sample block of a channel
-  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-      (void *)(planes + ((int64_t)lp * F + f) * KB * blk_bytes), (short)0,
+  const __amdgpu_buffer_rsrc_t r0 = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(J.planes + ((int64_t)lp * F + f) * KB * blk_bytes), (short)0,
       (int)(KB * blk_bytes), 0x00020000);
-  int voff[4];
+  const __amdgpu_buffer_rsrc_t rz = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)J.planes, (short)0, 0, 0x00020000);
+  const bool past = (ct + 1) * kTile >= Np;  // groups 8-11 lie past the array
+  int voff[6];
 #pragma unroll
-  for (int gg = 0; gg < 4; gg++) {
-    const int g = 4 * (wave & 1) + gg;
-    const int e0 = (g < 4 ? ti : tj) * kTile + 32 * (g & 3);
+  for (int gg = 0; gg < 6; gg++) {
+    const int g = 6 * gh + gg;
+    int e0 = g < 4 ? ti * kTile + 32 * g : ct * kTile + 32 * (g - 4);
+    if (e0 >= Np) e0 -= kTile;  // (loaded through the empty descriptor)
     voff[gg] = (e0 + (lane & 31)) * 16 + (lane >> 5) * (int)blk_bytes;
   }
-  auto issue_stage = [&](int st, int slot) {
-    const int kb0 = 4 * (st_begin + st);
+  auto issue = [&](int ks, int slot) {
+    const int soff = (k0 + ks) * 2 * (int)blk_bytes;
 #pragma unroll
-    for (int gg = 0; gg < 4; gg++)
-#pragma unroll
-      for (int k = 0; k < 2; k++) {
-        const int g = 4 * (wave & 1) + gg;
-        lds_void *dst = (lds_void *)&lds[slot][((lp * 8 + g) * 4 + 2 * k) * 512];
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, dst, 16, voff[gg],
-                                                 (kb0 + 2 * k) * (int)blk_bytes, 0, 0);
-      }
+    for (int gg = 0; gg < 6; gg++) {
+      const int g = 6 * gh + gg;
+      lds_void *dst = (lds_void *)&lds[slot][(lp * kGroups + g) * 1024];
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(g >= 8 && past ? rz : r0, dst, 16, voff[gg], soff,
+                                               0, 0);
+    }
   };
 
-  // Compute role: wave (wr, wc) = rows 64*wr.., columns 64*wc.. of the tile.
-  // 16 accumulators (RE split in two), so each is updated once per 16 MFMAs.
   const int wr = wave >> 1, wc = wave & 1;
-  v16i rea[2][2], reb[2][2], acc_ix[2][2], acc_iy[2][2];
+  v16i re[2][4], im[2][4];
   const v16i zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) rea[i][j] = reb[i][j] = acc_ix[i][j] = acc_iy[i][j] = zero;
-  struct Ops {
-    v4i ar[2], ai[2], br[2], bi[2];
+    for (int j = 0; j < 4; j++) re[i][j] = im[i][j] = zero;
+  // LDS read addresses: one lane/wave base per operand side; the slot,
+  // plane and group enter as immediate offsets.
+  const uint8_t *lds0 = &lds[0][0];
+  const int bA = lane * 16 + wr * 2048, bB = lane * 16 + 4096 + wc * 4096;
+  auto load_op = [&](int base, int slot, int plane, int k) -> v4i {
+    return *(const v4i *)(lds0 + base + (slot * kSlotBytes + plane * kGroups * 1024 + k * 1024));
   };
-  auto read_ops = [&](Ops &o, int slot, int k) {
+  v4i ar[2], ai[2], br[4], bi[4];  // this k-step's operands
+  auto read_a = [&](v4i (&xr)[2], v4i (&xi)[2], int slot) {
 #pragma unroll
     for (int i = 0; i < 2; i++) {
-      const int ga = 2 * wr + i, gb = 4 + 2 * wc + i;
-      o.ar[i] = *(const v4i *)&lds[slot][((0 * 8 + ga) * 4 + 2 * k) * 512 + lane * 16];
-      o.br[i] = *(const v4i *)&lds[slot][((0 * 8 + gb) * 4 + 2 * k) * 512 + lane * 16];
-      o.ai[i] = *(const v4i *)&lds[slot][((1 * 8 + ga) * 4 + 2 * k) * 512 + lane * 16];
-      o.bi[i] = *(const v4i *)&lds[slot][((1 * 8 + gb) * 4 + 2 * k) * 512 + lane * 16];
+      xr[i] = load_op(bA, slot, 0, i);
+      xi[i] = load_op(bA, slot, 1, i);
     }
   };
-  auto mfma_ops = [&](const Ops &o) {
+  // One k-step: for each column group j, its 8 MFMAs, after which that
+  // group's operands of the next k-step (slot nslot) replace it; the next
+  // k-step's row operands are read during group 0.
+  auto step_ops = [&](int nslot) {
+    v4i nai[2], nar[2], nxi[2];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 2; i++)  // -x per byte: bytes are multiples of 8
+      nai[i] = ((ai[i] ^ (int)0xF8F8F8F8u) + 0x08080808) & (int)0xF8F8F8F8u;
+    read_a(nar, nxi, nslot);
 #pragma unroll
-      for (int j = 0; j < 2; j++) {
-        rea[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(o.ar[i], o.br[j], rea[i][j], 0, 0, 0);
-        acc_iy[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(o.ar[i], o.bi[j], acc_iy[i][j], 0, 0, 0);
-        acc_ix[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(o.ai[i], o.br[j], acc_ix[i][j], 0, 0, 0);
-        reb[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(o.ai[i], o.bi[j], reb[i][j], 0, 0, 0);
+    for (int j = 0; j < 4; j++) {
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[i], br[j], re[i][j], 0, 0, 0);
+        im[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[i], bi[j], im[i][j], 0, 0, 0);
       }
-  };
-  int cur_mode = mode;
-  const int by0 = (ti * kTile + 64 * wr) / 32, bx0 = (tj * kTile + 64 * wc) / 32;
-
-  // Prologue: stages 0..kRing-2 in flight; wait for stage 0 (the 8 loads of
-  // each later stage may still be pending), read its first operands.
 #pragma unroll
-  for (int s = 0; s < kRing - 1; s++) issue_stage(s < nst ? s : nst - 1, s);
-  asm volatile("s_waitcnt vmcnt(16)\n\ts_barrier" ::: "memory");
-  Ops o0, o1;
-  read_ops(o0, 0, 0);
-
-  // Per stage st (slot st % kRing; the stage count is a multiple of kRing):
-  //   refill the slot of stage st-1 with stage st+kRing-1;
-  //   read k-step 1 of st, MFMAs of k-step 0;
-  //   wait for this wave's loads of st+1, barrier (all waves' loads landed,
-  //   all reads of the refilled slot done);
-  //   read k-step 0 of st+1, MFMAs of k-step 1.
-  const int stages_per_chunk = kChunk / kStage;
-  for (int c0 = 0; c0 < nst; c0 += stages_per_chunk) {
-    const int c1 = c0 + stages_per_chunk < nst ? c0 + stages_per_chunk : nst;
-    for (int st0 = c0; st0 < c1; st0 += kRing) {
-#pragma unroll
-      for (int d = 0; d < kRing; d++) {
-        const int st = st0 + d;
-        const int nx = st + kRing - 1;
-        issue_stage(nx < nst ? nx : nst - 1, (d + kRing - 1) % kRing);
-        read_ops(o1, d, 1);
-        mfma_ops(o0);
-        asm volatile("s_waitcnt vmcnt(16)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-        read_ops(o0, (d + 1) % kRing, 0);
-        mfma_ops(o1);
+      for (int i = 0; i < 2; i++) {
+        re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ai[i], bi[j], re[i][j], 0, 0, 0);
+        im[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(nai[i], br[j], im[i][j], 0, 0, 0);
       }
+      br[j] = load_op(bB, nslot, 0, j);
+      bi[j] = load_op(bB, nslot, 1, j);
     }
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 2; i++) {
+      ar[i] = nar[i];
+      ai[i] = nxi[i];
+    }
+  };
+  const int by0 = (ti * kTile + 64 * wr) / 32, bx0 = (ct * kTile + 128 * wc) / 32;
+
+  // Prologue: k-steps 0..kRing-1 in flight (6 loads each); wait for this
+  // wave's k-step 0, barrier, read it.
 #pragma unroll
-      for (int j = 0; j < 2; j++) {
-        const int by = by0 + i, bx = bx0 + j;
-        if (bx >= by && bx < nb) {
-          const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
-          flush_block(vis + ((int64_t)f * NB + b) * 2048, rea[i][j], reb[i][j], acc_ix[i][j],
-                      acc_iy[i][j], U, cur_mode, lane);
-        }
-        rea[i][j] = reb[i][j] = acc_ix[i][j] = acc_iy[i][j] = zero;
-      }
-    if (cur_mode == 0) cur_mode = 1;
+  for (int k = 0; k < kRing; k++) issue(k < nk ? k : nk - 1, k);
+  asm volatile("s_waitcnt vmcnt(18)\n\ts_barrier" ::: "memory");
+  read_a(ar, ai, 0);
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    br[j] = load_op(bB, 0, 0, j);
+    bi[j] = load_op(bB, 0, 1, j);
   }
-  // Drain the trailing (unused) loads: they must not land in a ring slot
-  // after the next tile's first loads.
+
+  // K-step ks (slot ks % kRing):
+  //   wait for this wave's loads of ks+1 (those of ks+2, ks+3 may stay
+  //   pending), barrier: every wave's loads of ks+1 landed; every wave's
+  //   reads of slot ks are done (they were waited for before the barrier);
+  //   refill slot ks with k-step ks+kRing; MFMAs of ks, reading ks+1's
+  //   operands as ks's die.
+  for (int k00 = 0; k00 < nk; k00 += kRing) {
+#pragma unroll
+    for (int d = 0; d < kRing; d++) {
+      const int ks = k00 + d;
+      const int nx = ks + kRing;
+      __builtin_amdgcn_sched_barrier(0);
+      asm volatile("s_waitcnt vmcnt(12)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+      __builtin_amdgcn_sched_barrier(0);
+      step_ops((d + 1) % kRing);
+      issue(nx < nk ? nx : nk - 1, d);
+      // group 0: 8 MFMAs with the 4 reads of the next row operands
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+      // groups 1-3: 2 reads (the previous group's next operands) and 2 DMAs
+      // over each group's 8 MFMAs
+#pragma unroll
+      for (int g = 0; g < 3; g++) {
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+          __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM (LDS DMA)
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+      }
+      __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // the last group's next operands
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  // Every partial sum is a multiple of 64 of magnitude <= 2 * 64 * 64 *
+  // 32 * (2 * kMaxItemStages) = 2^30: no int32 overflow.
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int by = by0 + i, bx = bx0 + j;
+      if (bx >= by && bx < nb) {
+        const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
+        flush_block(vis + ((int64_t)f * NB + b) * 2048, re[i][j], im[i][j], U, mode, lane);
+      }
+    }
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
-// Tile index tb (row-major over the upper triangle of ntiles x ntiles tiles)
-// -> (ti, tj), tj >= ti.
-__device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti, int &tj) {
-  int r = 0;
-  while (tb >= ntiles - r) {
-    tb -= ntiles - r;
-    r++;
-  }
-  ti = r;
-  tj = r + tb;
-}
-
-// One plan: grid.x = channels x upper-triangle tiles (XCD-remapped), grid.y
-// = time splits.
 // One plan chunk: items = channels x tiles.  Workgroups 0..head-1 take
 // items 0..head-1 whole (all stages, mode J.mode); the rest take the tail
 // items head.. cut into time pieces of J.stages_per_split stages (the last
 // piece the rest), mode tmode.  Each range is XCD-remapped.
 __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(GemmJob J, int nst, int head,
                                                            int tail, int tmode) {
-  __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kStageBytes];
-  const int64_t per_f = (int64_t)J.ntiles * (J.ntiles + 1) / 2;
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kSlotBytes];
+  const int64_t per_f = wide_tiles(J.ntiles);
   int64_t item;
   int s0, ns;
   if ((int)blockIdx.x < head) {
@@ -369,21 +411,18 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(GemmJob J, int nst, i
     ns = nst - s0 < J.stages_per_split ? nst - s0 : J.stages_per_split;
     J.mode = tmode;
   }
-  int ti, tj;
-  tile_coords((int)(item % per_f), J.ntiles, ti, tj);
-  gemm_tile(J, (int)(item / per_f), ti, tj, s0, ns, lds);
+  int ti, ct;
+  tile_coords((int)(item % per_f), J.ntiles, ti, ct);
+  gemm_tile(J, (int)(item / per_f), ti, ct, s0, ns, lds);
 }
 
-// Batch: persistent workgroups take work items (job, channel, tile, stage
-// range) from 8 queues, one per XCD (dispatch places workgroup i on XCD
-// i % 8), so that the tiles of one channel run on one XCD and share its L2.
-// A workgroup drains its own queue, then moves on to the next ones; queues
-// never refill, so every workgroup ends after at most nitems items and 8
-// queue moves.  Queue q holds items [qbound[q], qbound[q+1]).
+// Batch: persistent workgroups take work items from 8 per-XCD queues: a
+// workgroup on XCD x = blockIdx % 8 drains queue x, then moves on to the
+// others; queues never refill, so every workgroup ends.
 __global__ __launch_bounds__(256, 1) void corr_mfma_batch_kernel(
     const GemmJob *__restrict__ jobs, const GemmItem *__restrict__ items,
     const int *__restrict__ qbound, int *__restrict__ counters) {
-  __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kStageBytes];
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kSlotBytes];
   __shared__ int next;
   int q = blockIdx.x % 8, moves = 0;
   for (;;) {
@@ -406,9 +445,9 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_batch_kernel(
     if (it < 0) break;
     const GemmItem w = items[it];
     const GemmJob J = jobs[w.job];
-    int ti, tj;
-    tile_coords(w.tile, J.ntiles, ti, tj);
-    gemm_tile(J, w.f, ti, tj, w.stage0, w.nstages, lds);
+    int ti, ct;
+    tile_coords(w.tile, J.ntiles, ti, ct);
+    gemm_tile(J, w.f, ti, ct, w.stage0, w.nstages, lds);
   }
 }
 
@@ -423,10 +462,11 @@ static int64_t chunk_samples(const chx_plan *p) {
   // one (plane, channel) region must stay addressable by a buffer descriptor
   if (tc > 0x7FFFFFFFLL / Np) tc = 0x7FFFFFFFLL / Np;
   if (tc > 16LL * 65535) tc = 16LL * 65535;  // unpack grid.y = KB <= 65535
-  tc = tc / (kStage * kRing) * (kStage * kRing);
-  const int64_t T = (p->cfg.time_steps + kStage * kRing - 1) / (kStage * kRing) * (kStage * kRing);
+  tc = tc / (kMfmaStage * kMfmaRing) * (kMfmaStage * kMfmaRing);
+  const int64_t T = (p->cfg.time_steps + kMfmaStage * kMfmaRing - 1) / (kMfmaStage * kMfmaRing) *
+                    (kMfmaStage * kMfmaRing);
   if (tc > T) tc = T;
-  if (tc < kStage * kRing) tc = kStage * kRing;
+  if (tc < kMfmaStage * kMfmaRing) tc = kMfmaStage * kMfmaRing;
   return tc;
 }
 
@@ -440,27 +480,33 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
   const int N = p->cfg.num_elements, F = p->cfg.num_freq, U = p->cfg.upper_triangle_convention;
   const int64_t T = p->cfg.time_steps;
   const int ntiles = p->ntiles, Np = ntiles * kTile;
-  const int64_t items = (int64_t)F * ntiles * (ntiles + 1) / 2;
+  const int64_t per_f = wide_tiles(ntiles);
+  const int64_t items = (int64_t)F * per_f;
   if (items > 0x7FFFFFFF || !p->d_work) return CHX_ERR_SHAPE;
   int cus = 0;
   CHX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->cfg.device));
   if (cus < 1) cus = 1;
   const int64_t tc = chunk_samples(p);
   const int64_t row_bytes = (int64_t)F * N;
+  const int ring = kMfmaRing;  // stage counts are multiples of it
   for (int64_t t0 = 0; t0 < T; t0 += tc) {
-    const int64_t tv = T - t0 < tc ? T - t0 : tc;          // valid samples
-    // stages, padded to a multiple of the ring (the padding samples are 0)
-    const int nst_chunk = (int)((tv + kStage * kRing - 1) / (kStage * kRing)) * kRing;
-    const int KB = nst_chunk * (kStage / 16);
-    // Launch shape (one dispatch): whole rounds of one workgroup per CU take
-    // whole items with plain stores; the items of a last partial round are
-    // cut in time into the fewest pieces (whole rings, >= 16 stages) that
-    // minimise its length, adding atomically into zeroed output.
-    const int64_t head = items / cus * cus, tail = items - head;
+    const int64_t tv = T - t0 < tc ? T - t0 : tc;  // valid samples
+    // 64-sample stages, padded to a multiple of the ring (the padding is 0)
+    const int nst = (int)((tv + kMfmaStage * ring - 1) / (kMfmaStage * ring)) * ring;
+    const int KB = nst * (kMfmaStage / 16);
+    // Launch shape (one dispatch).  Whole rounds of one workgroup per CU
+    // take whole items (the chunk in one piece, plain stores); the items of
+    // a last partial round are cut in time into the fewest pieces that
+    // minimise its length and add atomically into zeroed output.  A chunk
+    // longer than kMaxItemStages cuts every item into pieces of at most
+    // kMaxItemStages stages (atomic).
+    int64_t head = 0;
     int split = 1;
+    if (nst <= kMaxItemStages) head = items / cus * cus;
+    const int64_t tail = items - head;
     if (tail > 0) {
       double best = (double)((tail + cus - 1) / cus);
-      for (int sp = 2; sp <= 64 && nst_chunk / sp >= 16; sp++) {
+      for (int sp = 2; sp <= 64 && nst / sp >= 16; sp++) {
         const double r = (double)((tail * sp + cus - 1) / cus) / sp;
         if (r < best - 1e-9) {
           best = r;
@@ -468,12 +514,13 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
         }
       }
     }
-    const int sps = ((nst_chunk + split - 1) / split + kRing - 1) / kRing * kRing;
-    split = (nst_chunk + sps - 1) / sps;
+    int sps = ((nst + split - 1) / split + ring - 1) / ring * ring;
+    if (sps > kMaxItemStages) sps = kMaxItemStages;
+    split = (nst + sps - 1) / sps;
     const int kmode = (t0 == 0) ? mode : 1;
     const int tmode = split > 1 ? 2 : kmode;
     if (tail > 0 && split > 1 && kmode == 0) {
-      const int64_t f0 = head / ((int64_t)ntiles * (ntiles + 1) / 2);
+      const int64_t f0 = head / per_f;  // zero the channels the tail touches
       int rc = launch_fill_u32((uint32_t *)vis + f0 * p->NB * 2048, 0u,
                                (int64_t)(F - f0) * p->NB * 2048, s);
       if (rc) return rc;
@@ -502,8 +549,8 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
     J.stages_per_split = sps;
     const int64_t grid = head + tail * split;
     if (grid > 0x7FFFFFFF) return CHX_ERR_SHAPE;
-    corr_mfma_kernel<<<dim3((unsigned)grid), dim3(256), 0, s>>>(J, nst_chunk, (int)head,
-                                                                  (int)tail, tmode);
+    corr_mfma_kernel<<<dim3((unsigned)grid), dim3(256), 0, s>>>(J, nst, (int)head, (int)tail,
+                                                                  tmode);
     CHX_HIP(hipGetLastError());
     if (p->prof) CHX_HIP(hipEventRecord(ev[2], s));
   }

@@ -161,16 +161,14 @@ def _fake(jobs):
 
 
 def _tiles(nt, method):
-    """Output tiles of one channel: int8 128 x 128 over the upper triangle;
-    fp6 128 x 256, row tile i holding ceil((nt - i) / 2) of them."""
-    if method == 2:
-        return nt * (nt + 1) // 2
+    """Output tiles of one channel (both matrix-core methods): 128 x 256, row
+    tile i holding ceil((nt - i) / 2) of them."""
     return sum((nt - i + 1) // 2 for i in range(nt))
 
 
 def _expected_items(jobs, chunk, max_st, unit, method):
     """Work items of the plan: per round, per active job, F x tiles x pieces
-    (fp6 pieces hold at most 2048 stages)."""
+    (pieces hold at most 2048 stages)."""
     n, rounds = 0, -(-max(j["time_steps"] for j in jobs) // chunk)
     for r in range(rounds):
         for j in jobs:
@@ -179,8 +177,7 @@ def _expected_items(jobs, chunk, max_st, unit, method):
                 continue
             nst = -(-min(chunk, j["time_steps"] - t0) // unit) * (unit // 64)
             per = max_st if 0 < max_st < nst else nst
-            if method == 3:
-                per = min(per, 2048)
+            per = min(per, 2048)
             nt = -(-j["num_elements"] // 128)
             n += j["num_freq"] * _tiles(nt, method) * -(-nst // per)
     return rounds, n
@@ -221,16 +218,15 @@ def test_batch_plan_host_only(tuning, method):
     assert info["method"] == method
 
 
-def test_fp6_items_hold_at_most_2048_stages():
-    """A long job is cut into FP6 work items of <= 2048 stages (FP32 partial
-    sums stay <= 2^24, where they are exact); int8 items are not cut."""
+@pytest.mark.parametrize("method", [2, 3])
+def test_items_hold_at_most_2048_stages(method):
+    """A long job is cut into work items of <= 2048 stages (FP6's FP32
+    partial sums stay <= 2^24, where they are exact; int8's int32 sums of 64x
+    products <= 2^30)."""
     job = [dict(num_elements=256, num_freq=3, time_steps=5000 * 64)]  # 5000 stages
-    big = dict(workspace_bytes=64 << 30)
-    f6 = chx.batch_plan(_fake(job), dict(big, method=3))
-    assert f6["rounds"] == 1 and f6["chunk_samples"] == 5000 * 64
-    assert f6["items"] == 3 * 2 * 3  # channels x tiles (nt = 2) x ceil(5000 / 2048) pieces
-    i8 = chx.batch_plan(_fake(job), dict(big, method=2))
-    assert i8["items"] == 3 * 3  # channels x tiles (nt = 2: 3 of 128 x 128), whole chunk
+    plan = chx.batch_plan(_fake(job), dict(workspace_bytes=64 << 30, method=method))
+    assert plan["rounds"] == 1 and plan["chunk_samples"] == 5000 * 64
+    assert plan["items"] == 3 * 2 * 3  # channels x tiles (nt = 2) x ceil(5000 / 2048) pieces
 
 
 def test_batch_plan_validates_like_create():
