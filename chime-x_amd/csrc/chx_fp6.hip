@@ -283,8 +283,7 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
 
   // Loads: wave w fills plane w/2, groups 6(w&1)..+5: six P0 operands and
   // three P1 group pairs per stage.  Group g < 4 holds rows kTile ti + 32 g,
-  // g >= 4 columns kTile ct + 32 (g - 4); a column half past the array
-  // reloads the first half (its products are never stored).
+  // g >= 4 columns kTile ct + 32 (g - 4).
   const int lp = wave >> 1, gh = wave & 1;
   const int64_t p0_blk = (int64_t)2 * Np * 16;  // one 64-sample block of one channel
   const int64_t p1_blk = (int64_t)2 * Np * 8;
@@ -396,7 +395,7 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   };
   const int by0 = (ti * kTile + 64 * wr) / 32, bx0 = (ct * kTile + 128 * wc) / 32;
 
-  // Prologue: stages 0..kRing-2 in flight (9 loads per stage); wait for this
+  // Prologue: stages 0..kRing-1 in flight (9 loads per stage); wait for this
   // wave's stage 0, barrier, read it.
 #pragma unroll
   for (int s = 0; s < kRing; s++) issue_stage(s < nst ? s : nst - 1, s);
@@ -409,11 +408,11 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   }
 
   // Stage st (slot st % kRing):
-  //   wait for this wave's loads of st+1 (those of st+2 may stay pending),
-  //   barrier: every wave's loads of st+1 landed; every wave's reads of the
-  //   slot of st-1 are done (they fed st-1's MFMAs);
-  //   MFMAs of st, reading st+1's operands as st's die; refill the slot of
-  //   st-1 with stage st+kRing-1, one DMA per two MFMAs.
+  //   wait for this wave's loads of st+1 (those of st+2, st+3 may stay
+  //   pending), barrier: every wave's loads of st+1 landed; every wave's
+  //   reads of slot st are done (they were waited for before the barrier);
+  //   refill slot st with stage st+kRing; MFMAs of st, reading st+1's
+  //   operands as st's die.
   for (int st0 = 0; st0 < nst; st0 += kRing) {
 #pragma unroll
     for (int d = 0; d < kRing; d++) {
