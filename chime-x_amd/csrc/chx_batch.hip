@@ -1,14 +1,16 @@
-// chx_batch.hip -- many int8 matrix-core correlations of uneven shape
-// (elements, channels, integration length) in one captured hipGraph.
+// chx_batch.hip -- many matrix-core correlations (int8 or FP6 operands) of
+// uneven shape (elements, channels, integration length) in one captured
+// hipGraph.
 //
 // Plan (host, at creation):
 //  * time is cut into rounds of chunk_samples samples, the same for every
-//    job, sized so that one round's int8 planes of all jobs still active fit
-//    the workspace budget; a job takes part in the rounds its length reaches;
+//    job, sized so that one round's planes of all jobs still active fit the
+//    workspace budget; a job takes part in the rounds its length reaches;
 //  * per round: one batched unpack launch over every active job's chunk, then
-//    one persistent GEMM launch over work items (job, channel, 128x128 tile,
-//    stage range); a job's tiles are cut in time when max_item_stages asks,
-//    in which case their partial sums add atomically;
+//    one persistent GEMM launch over work items (job, channel, 128x256 tile,
+//    stage range); a job's tiles are cut in time when max_item_stages asks
+//    or the chunk exceeds the item cap (kMaxItemStages), in which case their
+//    partial sums add atomically;
 //  * items are grouped per (job, channel) and the groups spread over 8
 //    per-XCD queues by longest-processing-time first, so a channel's tiles
 //    share one XCD's L2; inside a queue, long items run first.
