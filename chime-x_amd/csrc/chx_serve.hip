@@ -10,7 +10,8 @@
 //   one batched unpack of all frames (a stream's frames side by side in one
 //   plane range), then one persistent matrix-core pass with one GEMM job
 //   per stream, which adds the group's products into the stream's
-//   device-resident accumulator.
+//   device-resident accumulator (the last partial round of each XCD queue
+//   cut into time pieces that add atomically, so no workgroup idles).
 // Frames that arrive meanwhile are copied while the launch runs and form
 // the next launch.  Latency of a frame = submit -> completion of the launch
 // that integrated it, measured on the host clock.
