@@ -21,13 +21,17 @@
 //     of each element's 32-sample half of a 64-sample block:
 //       P0[plane][F][KB][half][Np][16 B],  P1[plane][F][KB][half][Np][8 B].
 //     A lane's MFMA operand (row l%32, samples 32(l/32)..+31 of the block)
-//     is then one 16-byte and one 8-byte LDS read at lane * 16 / lane * 8.
+//     is then one 16-byte and one 8-byte read at lane * 16 / lane * 8 of a
+//     32-element group (in LDS or in the planes).
 //  2. corr_f6_kernel: a workgroup = 4 waves = a 128-row x 256-column tile
-//     of one channel, each wave 64 x 128 (2x4 blocks x 2 FP32 accumulators
-//     = 256 AGPRs), fed by buffer->LDS DMA through a ring of 4 stages of one
-//     64-sample k-step (36 KB each), three stages in flight.  The load path
-//     (L2 -> LDS) bounds it: the wide tile moves 25% fewer bytes per product
-//     than a square 128 x 128 one.
+//     of one channel, each wave all 128 rows x 64 columns (4x2 blocks x 2
+//     FP32 accumulators = 256 AGPRs).  The row operands, shared by the four
+//     waves, come through a ring of 4 LDS stages of one 64-sample k-step
+//     (12 KB each) filled by buffer->LDS DMA; each wave loads its own
+//     column operands straight into VGPRs, three stages ahead, so every
+//     operand byte crosses the CU's load path once.  The load path bounds
+//     the kernel: each load instruction holds up the wave's in-order issue,
+//     and a DMA costs about 1.5x a direct load of the same bytes.
 #include "chx_internal.h"
 
 namespace chx {
@@ -45,9 +49,9 @@ constexpr int kTile = kMfmaTile;     // 128: tile rows; a tile has 2 * kTile col
 constexpr int kK = 64;               // samples per k-step (= per stage)
 constexpr int kRing = kFp6Ring;      // LDS stages in the ring
 constexpr int kMaxStages = kMaxItemStages;  // stages of one work item
-constexpr int kGroups = 12;          // 32-element groups per stage: 4 of rows, 8 of columns
-constexpr int kP0Bytes = 2 * kGroups * 1024;  // [plane 2][group 12][64 lanes][16 B]
-constexpr int kP1Bytes = 2 * kGroups * 512;   // [plane 2][group 12][64 lanes][8 B]
+constexpr int kRowGroups = 4;        // 32-element row groups per stage (in LDS)
+constexpr int kP0Bytes = 2 * kRowGroups * 1024;  // [plane 2][group 4][64 lanes][16 B]
+constexpr int kP1Bytes = 2 * kRowGroups * 512;   // [plane 2][group 4][64 lanes][8 B]
 constexpr int kSlotBytes = kP0Bytes + kP1Bytes;
 static_assert(kMaxStages % kRing == 0, "work items hold whole rings");
 
@@ -270,8 +274,11 @@ __device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti, int &ct
 
 // Pass 2, one output tile (rows from kTile ti, columns from kTile ct) of
 // channel f over k-steps [st_begin, st_begin + nst) (nst a multiple of
-// kRing).  Wave w computes rows 64 (w / 2).. x columns 128 (w % 2).. as 2x4
-// blocks of two FP32 accumulators (256 AGPRs).
+// kRing).  Wave w computes all 128 rows x columns 64 w.. as 4x2 blocks of
+// two FP32 accumulators (256 AGPRs).  Row operands (4 groups x 2 planes,
+// shared by the four waves) go through the LDS ring by buffer->LDS DMA;
+// each wave loads its own column operands (2 groups x 2 planes) straight
+// into VGPRs, three stages ahead.
 __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, const int ti,
                                              const int ct, const int st_begin, const int nst,
                                              uint8_t (*lds)[kSlotBytes]) {
@@ -281,180 +288,213 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
-  // Loads: wave w fills plane w/2, groups 6(w&1)..+5: six P0 operands and
-  // three P1 group pairs per stage.  Group g < 4 holds rows kTile ti + 32 g,
-  // g >= 4 columns kTile ct + 32 (g - 4).
-  const int lp = wave >> 1, gh = wave & 1;
   const int64_t p0_blk = (int64_t)2 * Np * 16;  // one 64-sample block of one channel
   const int64_t p1_blk = (int64_t)2 * Np * 8;
-  const __amdgpu_buffer_rsrc_t r0 = __builtin_amdgcn_make_buffer_rsrc(
-      (void *)(J.planes + ((int64_t)lp * F + f) * KB * p0_blk), (short)0, (int)(KB * p0_blk),
-      0x00020000);
-  const __amdgpu_buffer_rsrc_t r1 = __builtin_amdgcn_make_buffer_rsrc(
-      (void *)(J.planes1 + ((int64_t)lp * F + f) * KB * p1_blk), (short)0, (int)(KB * p1_blk),
-      0x00020000);
-  auto group_elem = [&](int g) {
-    int e = g < 4 ? ti * kTile + 32 * g : ct * kTile + 32 * (g - 4);
-    return e < Np ? e : e - kTile;
-  };
-  int voff0[6], voff1[3];
+  __amdgpu_buffer_rsrc_t r0[2], r1[2];
 #pragma unroll
-  for (int gg = 0; gg < 6; gg++)
-    voff0[gg] = ((lane >> 5) * Np + group_elem(6 * gh + gg) + (lane & 31)) * 16;
-#pragma unroll
-  for (int pp = 0; pp < 3; pp++) {
-    const int g = 6 * gh + 2 * pp + (lane >> 5);  // lanes 32-63: the pair's second group
-    const int i = lane & 31;
-    voff1[pp] = ((i >> 4) * Np + group_elem(g) + 2 * (i & 15)) * 8;
+  for (int pl = 0; pl < 2; pl++) {
+    r0[pl] = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(J.planes + ((int64_t)pl * F + f) * KB * p0_blk), (short)0, (int)(KB * p0_blk),
+        0x00020000);
+    r1[pl] = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(J.planes1 + ((int64_t)pl * F + f) * KB * p1_blk), (short)0, (int)(KB * p1_blk),
+        0x00020000);
   }
-  // Groups past the array (the right half of a row's last tile when the
-  // tile count is odd) load through an empty descriptor: every offset is
-  // out of range, so nothing is fetched (their products are never stored).
-  const __amdgpu_buffer_rsrc_t rz = __builtin_amdgcn_make_buffer_rsrc(
-      (void *)J.planes, (short)0, 0, 0x00020000);
-  const bool past = (ct + 1) * kTile >= Np;  // groups 8-11 lie past the array
-  auto issue_stage = [&](int st, int slot) {
+  // Row DMAs: wave w fills plane w/2, row groups 2(w&1), 2(w&1)+1: two P0
+  // operands and one P1 group pair per stage.  Row group g holds elements
+  // kTile ti + 32 g (always inside the array).
+  const int lp = wave >> 1, gh = wave & 1;
+  int voff0[2], voff1;
+#pragma unroll
+  for (int gg = 0; gg < 2; gg++)
+    voff0[gg] = ((lane >> 5) * Np + ti * kTile + 32 * (2 * gh + gg) + (lane & 31)) * 16;
+  {
+    const int g = 2 * gh + (lane >> 5);  // lanes 32-63: the pair's second group
+    const int i = lane & 31;
+    voff1 = ((i >> 4) * Np + ti * kTile + 32 * g + 2 * (i & 15)) * 8;
+  }
+  auto issue_rows = [&](int st, int slot) {
     const int kb = st_begin + st;
 #pragma unroll
-    for (int gg = 0; gg < 6; gg++) {
-      const int g = 6 * gh + gg;
-      lds_void *dst = (lds_void *)&lds[slot][(lp * kGroups + g) * 1024];
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(g >= 8 && past ? rz : r0, dst, 16, voff0[gg],
-                                               kb * (int)p0_blk, 0, 0);
+    for (int gg = 0; gg < 2; gg++) {
+      lds_void *dst = (lds_void *)&lds[slot][(lp * kRowGroups + 2 * gh + gg) * 1024];
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(r0[lp], dst, 16, voff0[gg], kb * (int)p0_blk, 0,
+                                               0);
     }
+    lds_void *dst = (lds_void *)&lds[slot][kP0Bytes + (lp * kRowGroups + 2 * gh) * 512];
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r1[lp], dst, 16, voff1, kb * (int)p1_blk, 0, 0);
+  };
+  // Column operands of this wave: groups 2 w + j, lane -> (half lane/32,
+  // element + lane%32).  Columns past the array (the right half of a row's
+  // last tile when the tile count is odd: waves 2 and 3) load through an
+  // empty descriptor, which fetches nothing and returns zeros; their
+  // products are never stored.
+  const bool past = wave >= 2 && (ct + 1) * kTile >= Np;
+  const __amdgpu_buffer_rsrc_t rz = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)J.planes, (short)0, 0, 0x00020000);
+  __amdgpu_buffer_rsrc_t c0[2], c1[2];
 #pragma unroll
-    for (int pp = 0; pp < 3; pp++) {
-      const int g = 6 * gh + 2 * pp;
-      lds_void *dst = (lds_void *)&lds[slot][kP0Bytes + (lp * kGroups + g) * 512];
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(g >= 8 && past ? rz : r1, dst, 16, voff1[pp],
-                                               kb * (int)p1_blk, 0, 0);
-    }
+  for (int pl = 0; pl < 2; pl++) {
+    c0[pl] = past ? rz : r0[pl];
+    c1[pl] = past ? rz : r1[pl];
+  }
+  int boff0[2], boff1[2];
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    int e = ct * kTile + 64 * wave + 32 * j;
+    if (e >= Np) e -= kTile;
+    e += lane & 31;
+    boff0[j] = ((lane >> 5) * Np + e) * 16;
+    boff1[j] = ((lane >> 5) * Np + e) * 8;
+  }
+  auto load_op_g = [&](int pl, int j, int s0, int s1) -> v8i {
+    const v4u x = __builtin_amdgcn_raw_buffer_load_b128(c0[pl], boff0[j], s0, 0);
+    const v2u y = __builtin_amdgcn_raw_buffer_load_b64(c1[pl], boff1[j], s1, 0);
+    const v4u y4 = __builtin_shufflevector(y, y, 0, 1, -1, -1);
+    return (v8i)__builtin_shufflevector(x, y4, 0, 1, 2, 3, 4, 5, -1, -1);
+  };
+  auto load_cols = [&](v8i &R0, v8i &R1, v8i &I0, v8i &I1, int st) {
+    const int kb = st_begin + st;
+    const int s0 = kb * (int)p0_blk, s1 = kb * (int)p1_blk;
+    R0 = load_op_g(0, 0, s0, s1);
+    R1 = load_op_g(0, 1, s0, s1);
+    I0 = load_op_g(1, 0, s0, s1);
+    I1 = load_op_g(1, 1, s0, s1);
   };
 
-  const int wr = wave >> 1, wc = wave & 1;
-  v16f re[2][4], im[2][4];
+  v16f re[4][2], im[4][2];
   const v16f zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < 4; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) re[i][j] = im[i][j] = zero;
-  // LDS read addresses: one lane/wave base per (operand side, array); the
-  // slot, plane and group enter as immediate offsets.
+    for (int j = 0; j < 2; j++) re[i][j] = im[i][j] = zero;
   const uint8_t *lds0 = &lds[0][0];
-  const int bA0 = lane * 16 + wr * 2048, bB0 = lane * 16 + 4096 + wc * 4096;
-  const int bA1 = kP0Bytes + lane * 8 + wr * 1024, bB1 = kP0Bytes + lane * 8 + 2048 + wc * 2048;
-  auto load_op = [&](int b0, int b1, int slot, int plane, int k) -> v8i {
-    const v4i a = *(const v4i *)(lds0 + b0 + (slot * kSlotBytes + plane * kGroups * 1024 + k * 1024));
-    const v2i b = *(const v2i *)(lds0 + b1 + (slot * kSlotBytes + plane * kGroups * 512 + k * 512));
+  const int bA0 = lane * 16, bA1 = kP0Bytes + lane * 8;
+  auto load_op = [&](int slot, int plane, int k) -> v8i {
+    const v4i a = *(const v4i *)(lds0 + bA0 + (slot * kSlotBytes + plane * kRowGroups * 1024 + k * 1024));
+    const v2i b = *(const v2i *)(lds0 + bA1 + (slot * kSlotBytes + plane * kRowGroups * 512 + k * 512));
     return (v8i){a.x, a.y, a.z, a.w, b.x, b.y, 0, 0};
   };
-  // The sign bit (bit 5) of each of the 32 codes of an operand (16 codes per
-  // 3 words): flipping it negates the value exactly.
   const v8i sign = {0x20820820, 0x08208208, (int)0x82082082u, 0x20820820, 0x08208208,
                     (int)0x82082082u, 0, 0};
-  v8i ar[2], ai[2], br[4], bi[4];  // this stage's operands
-  auto read_a = [&](v8i (&xr)[2], v8i (&xi)[2], int slot) {
+  // Row operands of the current stage (row group 3 double-buffered: its
+  // next operands are read before its MFMAs end).
+  v8i ar[4], ai[4], ar3n, ai3n;
+  // One stage with column operands (br0, br1, bi0, bi1): for each row group
+  // i its 8 MFMAs, after which that group's operands of the next stage
+  // (slot nslot) replace it.
+  auto stage_ops = [&](const v8i &br0, const v8i &br1, const v8i &bi0, const v8i &bi1,
+                       int nslot) {
+    const v8i br[2] = {br0, br1}, bi[2] = {bi0, bi1};
 #pragma unroll
-    for (int i = 0; i < 2; i++) {
-      xr[i] = load_op(bA0, bA1, slot, 0, i);
-      xi[i] = load_op(bA0, bA1, slot, 1, i);
-    }
-  };
-  // One stage: for each column group j, its 8 MFMAs, after which that
-  // group's operands of the next stage (slot nslot) replace it; the next
-  // stage's row operands are read during group 0.
-  auto stage_ops = [&](int nslot) {
-    v8i nai[2], nar[2], nxi[2];
+    for (int i = 0; i < 4; i++) {
+      const v8i xr = i == 3 ? ar3n : ar[i], xi = i == 3 ? ai3n : ai[i];
+      const v8i nxi = xi ^ sign;
 #pragma unroll
-    for (int i = 0; i < 2; i++) nai[i] = ai[i] ^ sign;
-    read_a(nar, nxi, nslot);
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-#pragma unroll
-      for (int i = 0; i < 2; i++) {
-        re[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ar[i], br[j], re[i][j], 3, 3,
+      for (int j = 0; j < 2; j++) {
+        re[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(xr, br[j], re[i][j], 3, 3,
                                                                    0, 127, 0, 127);
-        im[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ar[i], bi[j], im[i][j], 3, 3,
+        im[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(xr, bi[j], im[i][j], 3, 3,
                                                                    0, 127, 0, 127);
       }
 #pragma unroll
-      for (int i = 0; i < 2; i++) {
-        re[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ai[i], bi[j], re[i][j], 3, 3,
+      for (int j = 0; j < 2; j++) {
+        re[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(xi, bi[j], re[i][j], 3, 3,
                                                                    0, 127, 0, 127);
-        im[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(nai[i], br[j], im[i][j], 3, 3,
+        im[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(nxi, br[j], im[i][j], 3, 3,
                                                                    0, 127, 0, 127);
       }
-      br[j] = load_op(bB0, bB1, nslot, 0, j);
-      bi[j] = load_op(bB0, bB1, nslot, 1, j);
+      if (i < 3) {
+        ar[i] = load_op(nslot, 0, i);
+        ai[i] = load_op(nslot, 1, i);
+      }
+      if (i == 1) {  // the next stage's row group 3, into the second buffer
+        ar[3] = load_op(nslot, 0, 3);
+        ai[3] = load_op(nslot, 1, 3);
+      }
     }
-#pragma unroll
-    for (int i = 0; i < 2; i++) {
-      ar[i] = nar[i];
-      ai[i] = nxi[i];
-    }
+    ar3n = ar[3];
+    ai3n = ai[3];
   };
-  const int by0 = (ti * kTile + 64 * wr) / 32, bx0 = (ct * kTile + 128 * wc) / 32;
+  const int by0 = ti * kTile / 32, bx0 = (ct * kTile + 64 * wave) / 32;
 
-  // Prologue: stages 0..kRing-1 in flight (9 loads per stage); wait for this
-  // wave's stage 0, barrier, read it.
+  // Issue groups: G(st), issued during stage st, = the row DMAs of stage
+  // st+4 (3) and the column loads of stage st+3 (8).  Prologue: G(-4) =
+  // DMAs of stage 0, G(-3..-1) = DMAs of stages 1..3 and column loads of
+  // 0..2 (sched barriers keep the issue order the waits count on).  Wait
+  // for this wave's row DMAs of stage 0 (33 later operations may stay
+  // pending), barrier, read them.  Column operands of stage s live in set
+  // s % 4: stage st loads st+3 into the set stage st-1 has finished.
+  static_assert(kRing == 4, "column loads three stages ahead, row DMAs four");
+  auto cl = [&](int s) { return s < nst ? s : nst - 1; };
+  v8i BR[4][2], BI[4][2];
+  issue_rows(0, 0);
+  __builtin_amdgcn_sched_barrier(0);
+  issue_rows(cl(1), 1);
+  load_cols(BR[0][0], BR[0][1], BI[0][0], BI[0][1], 0);
+  __builtin_amdgcn_sched_barrier(0);
+  issue_rows(cl(2), 2);
+  load_cols(BR[1][0], BR[1][1], BI[1][0], BI[1][1], cl(1));
+  __builtin_amdgcn_sched_barrier(0);
+  issue_rows(cl(3), 3);
+  load_cols(BR[2][0], BR[2][1], BI[2][0], BI[2][1], cl(2));
+  __builtin_amdgcn_sched_barrier(0);
+  asm volatile("s_waitcnt vmcnt(33)\n\ts_barrier" ::: "memory");
 #pragma unroll
-  for (int s = 0; s < kRing; s++) issue_stage(s < nst ? s : nst - 1, s);
-  asm volatile("s_waitcnt vmcnt(27)\n\ts_barrier" ::: "memory");
-  read_a(ar, ai, 0);
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    br[j] = load_op(bB0, bB1, 0, 0, j);
-    bi[j] = load_op(bB0, bB1, 0, 1, j);
+  for (int i = 0; i < 3; i++) {
+    ar[i] = load_op(0, 0, i);
+    ai[i] = load_op(0, 1, i);
   }
+  ar3n = load_op(0, 0, 3);
+  ai3n = load_op(0, 1, 3);
 
   // Stage st (slot st % kRing):
-  //   wait for this wave's loads of st+1 (those of st+2, st+3 may stay
-  //   pending), barrier: every wave's loads of st+1 landed; every wave's
-  //   reads of slot st are done (they were waited for before the barrier);
-  //   refill slot st with stage st+kRing; MFMAs of st, reading st+1's
-  //   operands as st's die.
+  //   wait until G(st-3) landed (this wave's DMAs of st+1 and column loads
+  //   of st; G(st-2), G(st-1) may stay pending), barrier: every wave's DMAs
+  //   of st+1 landed and every wave's reads of slot st are done; refill
+  //   slot st with stage st+4, load the columns of st+3; MFMAs of st,
+  //   reading st+1's row operands as st's die.
   for (int st0 = 0; st0 < nst; st0 += kRing) {
 #pragma unroll
     for (int d = 0; d < kRing; d++) {
-      const int st = st0 + d;
-      const int nx = st + kRing;
+      const int st = st0 + d, n3 = (d + 3) % kRing;
       __builtin_amdgcn_sched_barrier(0);
-      asm volatile("s_waitcnt vmcnt(18)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+      asm volatile("s_waitcnt vmcnt(22)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
       __builtin_amdgcn_sched_barrier(0);
-      stage_ops((d + 1) % kRing);
-      issue_stage(nx < nst ? nx : nst - 1, d);
-      // group 0: 8 MFMAs with the 8 reads of the next row operands
+      issue_rows(cl(st + 4), d);
+      load_cols(BR[n3][0], BR[n3][1], BI[n3][0], BI[n3][1], cl(st + 3));
+      stage_ops(BR[d][0], BR[d][1], BI[d][0], BI[d][1], (d + 1) % kRing);
+      // 8 MFMAs per row group i; VMEM ops (3 DMAs, 8 column loads) over
+      // groups 0-2, LDS reads of the next row operands from group 1 on
 #pragma unroll
       for (int i = 0; i < 8; i++) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
-        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
-      }
-      // groups 1-3: 4 reads (the previous group's next operands) and 3
-      // DMAs over each group's 8 MFMAs
-#pragma unroll
-      for (int g = 0; g < 3; g++) {
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-        }
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-          __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM (LDS DMA)
-        }
         __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (i < 4) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
       }
-      __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);  // the last group's next operands
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (i < 4) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        if (i >= 4) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+      }
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (i < 8) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        if (i < 3) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+      }
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (i < 4) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      }
       __builtin_amdgcn_sched_barrier(0);
     }
   }
-  // Every partial sum is an integer of magnitude <= 128 * 64 * kMaxStages =
-  // 2^24: exact in FP32.
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < 4; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < 2; j++) {
       const int by = by0 + i, bx = bx0 + j;
       if (bx >= by && bx < nb) {
         const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
