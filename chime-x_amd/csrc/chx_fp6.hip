@@ -539,8 +539,11 @@ __global__ __launch_bounds__(256, 1) void corr_f6_kernel(GemmJob J, int head, in
 __global__ __launch_bounds__(256, 1) void corr_f6_batch_kernel(
     const GemmJob *__restrict__ jobs, const GemmItem *__restrict__ items,
     const int *__restrict__ qbound, int *__restrict__ counters) {
-  __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kSlotBytes];
-  __shared__ int next;
+  // One LDS object: the ring and, past it, the item handed to the
+  // workgroup.  (A second __shared__ variable makes the compiler wait for
+  // every buffer->LDS load before each LDS read in the loop.)
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing + 1][kSlotBytes];
+  int &next = *(int *)&lds[kRing][0];
   int q = blockIdx.x % 8, moves = 0;
   for (;;) {
     if (threadIdx.x == 0) {

@@ -18,11 +18,12 @@
 //     its sign extended into bit 7 it is 8*value exactly.  Elements past N
 //     and samples past T are 0.
 //  2. corr_mfma_kernel: a workgroup = 4 wavefronts = a 128-row x 256-column
-//     tile of one channel, each wave 64 x 128 (2x4 blocks x 2 int32
-//     accumulators: RE = Ar.Br + Ai.Bi and IM = Ar.Bi + (-Ai).Br, -Ai
-//     negated per byte in registers), fed by buffer loads that go straight
-//     to LDS through a ring of 4 k-steps of 32 samples (24 KB each), three
-//     in flight.
+//     tile of one channel, each wave all 128 rows x 64 columns (4x2 blocks
+//     x 2 int32 accumulators: RE = Ar.Br + Ai.Bi and IM = Ar.Bi + (-Ai).Br,
+//     -Ai negated per byte in registers).  The row operands come through a
+//     ring of 4 LDS k-steps of 32 samples (8 KB each) filled by buffer->LDS
+//     DMA; each wave loads its own column operands straight into VGPRs,
+//     three k-steps ahead.
 // Every product is 64x the true one; a work item covers at most
 // kMaxItemStages 64-sample stages, so every accumulator stays a multiple of
 // 64 of magnitude <= 2^30, divided by 64 (exact) when flushed to the int32
@@ -40,12 +41,13 @@ typedef __attribute__((address_space(3))) void lds_void;
 
 constexpr int kTile = kMfmaTile;  // 128: tile rows; a tile has 2 * kTile columns
 constexpr int kRing = 4;           // LDS slots of one 32-sample k-step
-constexpr int kGroups = 12;        // 32-element groups per k-step: 4 of rows, 8 of columns
-// LDS k-step slot: [plane 2][group 12][1 KB operand].  A 32-row, 32-sample
-// operand is 1 KB contiguous: lanes 0-31 read the k-step's even 16-sample
-// block, lanes 32-63 the odd one, 16 lanes = 256 contiguous bytes (no bank
-// conflicts); one 64-lane direct load fills exactly one operand.
-constexpr int kSlotBytes = 2 * kGroups * 1024;  // 24 KB
+constexpr int kRowGroups = 4;     // 32-element row groups per k-step (in LDS)
+// LDS k-step slot: [plane 2][row group 4][1 KB operand].  A 32-row,
+// 32-sample operand is 1 KB contiguous: lanes 0-31 read the k-step's even
+// 16-sample block, lanes 32-63 the odd one, 16 lanes = 256 contiguous bytes
+// (no bank conflicts); one 64-lane buffer->LDS load fills exactly one
+// operand.
+constexpr int kSlotBytes = 2 * kRowGroups * 1024;  // 8 KB
 
 // 4x4 byte transpose: r[s] holds byte e of sample s; c[e] gets byte s of
 // element e.  v_perm_b32 selects bytes of {s0:s1} (selector 0-3 = s1,
@@ -216,11 +218,14 @@ __device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti, int &ct
 
 // Pass 2, one output tile (rows from kTile ti, columns from kTile ct) of
 // channel f over 64-sample stages [st_begin, st_begin + nst) (nst a multiple
-// of kMfmaRing), run as 32-sample k-steps.  Wave w computes rows 64 (w / 2)..
-// x columns 128 (w % 2).. as 2x4 blocks of two int32 accumulators (256
-// AGPRs):  RE += Ar.Br + Ai.Bi,  IM += Ar.Bi + (-Ai).Br,  where -Ai is
-// negated per byte in registers (exact: every byte is 8 x value, in
-// [-64, 56]).
+// of kMfmaRing), run as 32-sample k-steps.  Wave w computes all 128 rows x
+// columns 64 w.. as 4x2 blocks of two int32 accumulators (256 AGPRs):
+// RE += Ar.Br + Ai.Bi,  IM += Ar.Bi + (-Ai).Br,  where -Ai is negated per
+// byte in registers (exact: every byte is 8 x value, in [-64, 56]).  The
+// row operands, shared by the four waves, come through the LDS ring by
+// buffer->LDS DMA; each wave loads its own column operands straight into
+// VGPRs, three k-steps ahead, so every operand byte crosses the CU's load
+// path once.
 __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const int ti,
                                           const int ct, const int st_begin, const int nst,
                                           uint8_t (*lds)[kSlotBytes]) {
@@ -231,155 +236,181 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int k0 = 2 * st_begin, nk = 2 * nst;  // k-steps
 
-  // Loads: wave w fills plane w/2, groups 6(w&1)..+5 of each k-step (one
-  // 1 KB operand each: lanes 0-31 the k-step's even 16-sample block, 32-63
-  // the odd one).  Group g < 4 holds rows kTile ti + 32 g, g >= 4 columns
-  // kTile ct + 32 (g - 4); groups past the array (the right half of a row's
-  // last tile when the tile count is odd) load through an empty descriptor:
-  // every offset is out of range, nothing is fetched, their products are
-  // never stored.
-  const int lp = wave >> 1, gh = wave & 1;
+  // An operand (32 elements x 32 samples) is 1 KB: lanes 0-31 the k-step's
+  // even 16-sample block, 32-63 the odd one.
   const int64_t blk_bytes = (int64_t)Np * 16;  // one 16-sample block of a channel
-  const __amdgpu_buffer_rsrc_t r0 = __builtin_amdgcn_make_buffer_rsrc(
-      (void *)(J.planes + ((int64_t)lp * F + f) * KB * blk_bytes), (short)0,
-      (int)(KB * blk_bytes), 0x00020000);
-  const __amdgpu_buffer_rsrc_t rz = __builtin_amdgcn_make_buffer_rsrc(
-      (void *)J.planes, (short)0, 0, 0x00020000);
-  const bool past = (ct + 1) * kTile >= Np;  // groups 8-11 lie past the array
-  int voff[6];
+  __amdgpu_buffer_rsrc_t r[2];
 #pragma unroll
-  for (int gg = 0; gg < 6; gg++) {
-    const int g = 6 * gh + gg;
-    int e0 = g < 4 ? ti * kTile + 32 * g : ct * kTile + 32 * (g - 4);
-    if (e0 >= Np) e0 -= kTile;  // (loaded through the empty descriptor)
-    voff[gg] = (e0 + (lane & 31)) * 16 + (lane >> 5) * (int)blk_bytes;
-  }
-  auto issue = [&](int ks, int slot) {
+  for (int pl = 0; pl < 2; pl++)
+    r[pl] = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(J.planes + ((int64_t)pl * F + f) * KB * blk_bytes), (short)0,
+        (int)(KB * blk_bytes), 0x00020000);
+  // Row DMAs: wave w fills plane w/2, row groups 2(w&1), 2(w&1)+1 of each
+  // k-step.  Row group g holds elements kTile ti + 32 g (always inside the
+  // array).
+  const int lp = wave >> 1, gh = wave & 1;
+  int voff[2];
+#pragma unroll
+  for (int gg = 0; gg < 2; gg++)
+    voff[gg] = (ti * kTile + 32 * (2 * gh + gg) + (lane & 31)) * 16 + (lane >> 5) * (int)blk_bytes;
+  auto issue_rows = [&](int ks, int slot) {
     const int soff = (k0 + ks) * 2 * (int)blk_bytes;
 #pragma unroll
-    for (int gg = 0; gg < 6; gg++) {
-      const int g = 6 * gh + gg;
-      lds_void *dst = (lds_void *)&lds[slot][(lp * kGroups + g) * 1024];
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(g >= 8 && past ? rz : r0, dst, 16, voff[gg], soff,
-                                               0, 0);
+    for (int gg = 0; gg < 2; gg++) {
+      lds_void *dst = (lds_void *)&lds[slot][(lp * kRowGroups + 2 * gh + gg) * 1024];
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(r[lp], dst, 16, voff[gg], soff, 0, 0);
     }
   };
+  // Column operands of this wave: groups 2 w + j.  Columns past the array
+  // (the right half of a row's last tile when the tile count is odd: waves
+  // 2 and 3) load through an empty descriptor, which fetches nothing and
+  // returns zeros; their products are never stored.
+  const bool past = wave >= 2 && (ct + 1) * kTile >= Np;
+  const __amdgpu_buffer_rsrc_t rz = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)J.planes, (short)0, 0, 0x00020000);
+  const __amdgpu_buffer_rsrc_t c0 = past ? rz : r[0], c1 = past ? rz : r[1];
+  int boff[2];
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    int e = ct * kTile + 64 * wave + 32 * j;
+    if (e >= Np) e -= kTile;
+    boff[j] = (e + (lane & 31)) * 16 + (lane >> 5) * (int)blk_bytes;
+  }
+  auto load_cols = [&](v4i &R0, v4i &R1, v4i &I0, v4i &I1, int ks) {
+    const int soff = (k0 + ks) * 2 * (int)blk_bytes;
+    R0 = (v4i)__builtin_amdgcn_raw_buffer_load_b128(c0, boff[0], soff, 0);
+    R1 = (v4i)__builtin_amdgcn_raw_buffer_load_b128(c0, boff[1], soff, 0);
+    I0 = (v4i)__builtin_amdgcn_raw_buffer_load_b128(c1, boff[0], soff, 0);
+    I1 = (v4i)__builtin_amdgcn_raw_buffer_load_b128(c1, boff[1], soff, 0);
+  };
 
-  const int wr = wave >> 1, wc = wave & 1;
-  v16i re[2][4], im[2][4];
+  v16i re[4][2], im[4][2];
   const v16i zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < 4; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) re[i][j] = im[i][j] = zero;
-  // LDS read addresses: one lane/wave base per operand side; the slot,
-  // plane and group enter as immediate offsets.
+    for (int j = 0; j < 2; j++) re[i][j] = im[i][j] = zero;
   const uint8_t *lds0 = &lds[0][0];
-  const int bA = lane * 16 + wr * 2048, bB = lane * 16 + 4096 + wc * 4096;
-  auto load_op = [&](int base, int slot, int plane, int k) -> v4i {
-    return *(const v4i *)(lds0 + base + (slot * kSlotBytes + plane * kGroups * 1024 + k * 1024));
+  const int bA = lane * 16;
+  auto load_op = [&](int slot, int plane, int k) -> v4i {
+    return *(const v4i *)(lds0 + bA + (slot * kSlotBytes + plane * kRowGroups * 1024 + k * 1024));
   };
-  v4i ar[2], ai[2], br[4], bi[4];  // this k-step's operands
-  auto read_a = [&](v4i (&xr)[2], v4i (&xi)[2], int slot) {
+  // Row operands of the current k-step (row group 3 double-buffered: its
+  // next operands are read before its MFMAs end).
+  v4i ar[4], ai[4], ar3n, ai3n;
+  // One k-step with column operands (br0, br1, bi0, bi1): for each row
+  // group i its 8 MFMAs, after which that group's operands of the next
+  // k-step (slot nslot) replace it.
+  auto step_ops = [&](const v4i &br0, const v4i &br1, const v4i &bi0, const v4i &bi1,
+                      int nslot) {
+    const v4i br[2] = {br0, br1}, bi[2] = {bi0, bi1};
 #pragma unroll
-    for (int i = 0; i < 2; i++) {
-      xr[i] = load_op(bA, slot, 0, i);
-      xi[i] = load_op(bA, slot, 1, i);
-    }
-  };
-  // One k-step: for each column group j, its 8 MFMAs, after which that
-  // group's operands of the next k-step (slot nslot) replace it; the next
-  // k-step's row operands are read during group 0.
-  auto step_ops = [&](int nslot) {
-    v4i nai[2], nar[2], nxi[2];
+    for (int i = 0; i < 4; i++) {
+      const v4i xr = i == 3 ? ar3n : ar[i], xi = i == 3 ? ai3n : ai[i];
+      // -x per byte: bytes are multiples of 8
+      const v4i nxi = ((xi ^ (int)0xF8F8F8F8u) + 0x08080808) & (int)0xF8F8F8F8u;
 #pragma unroll
-    for (int i = 0; i < 2; i++)  // -x per byte: bytes are multiples of 8
-      nai[i] = ((ai[i] ^ (int)0xF8F8F8F8u) + 0x08080808) & (int)0xF8F8F8F8u;
-    read_a(nar, nxi, nslot);
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-#pragma unroll
-      for (int i = 0; i < 2; i++) {
-        re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[i], br[j], re[i][j], 0, 0, 0);
-        im[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ar[i], bi[j], im[i][j], 0, 0, 0);
+      for (int j = 0; j < 2; j++) {
+        re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(xr, br[j], re[i][j], 0, 0, 0);
+        im[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(xr, bi[j], im[i][j], 0, 0, 0);
       }
 #pragma unroll
-      for (int i = 0; i < 2; i++) {
-        re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ai[i], bi[j], re[i][j], 0, 0, 0);
-        im[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(nai[i], br[j], im[i][j], 0, 0, 0);
+      for (int j = 0; j < 2; j++) {
+        re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(xi, bi[j], re[i][j], 0, 0, 0);
+        im[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(nxi, br[j], im[i][j], 0, 0, 0);
       }
-      br[j] = load_op(bB, nslot, 0, j);
-      bi[j] = load_op(bB, nslot, 1, j);
+      if (i < 3) {
+        ar[i] = load_op(nslot, 0, i);
+        ai[i] = load_op(nslot, 1, i);
+      }
+      if (i == 1) {  // the next k-step's row group 3, into the second buffer
+        ar[3] = load_op(nslot, 0, 3);
+        ai[3] = load_op(nslot, 1, 3);
+      }
     }
-#pragma unroll
-    for (int i = 0; i < 2; i++) {
-      ar[i] = nar[i];
-      ai[i] = nxi[i];
-    }
+    ar3n = ar[3];
+    ai3n = ai[3];
   };
-  const int by0 = (ti * kTile + 64 * wr) / 32, bx0 = (ct * kTile + 128 * wc) / 32;
+  const int by0 = ti * kTile / 32, bx0 = (ct * kTile + 64 * wave) / 32;
 
-  // Prologue: k-steps 0..kRing-1 in flight (6 loads each); wait for this
-  // wave's k-step 0, barrier, read it.
-#pragma unroll
-  for (int k = 0; k < kRing; k++) issue(k < nk ? k : nk - 1, k);
+  // Issue groups: G(ks), issued during k-step ks, = the row DMAs of k-step
+  // ks+4 (2) and the column loads of ks+3 (4).  Prologue: G(-4) = DMAs of
+  // k-step 0, G(-3..-1) = DMAs of 1..3 and column loads of 0..2 (sched
+  // barriers keep the issue order the waits count on).  Wait for this
+  // wave's row DMAs of k-step 0 (18 later operations may stay pending),
+  // barrier, read them.  Column operands of k-step s live in set s % 4:
+  // k-step ks loads ks+3 into the set k-step ks-1 has finished.
+  static_assert(kRing == 4, "column loads three k-steps ahead, row DMAs four");
+  auto cl = [&](int s) { return s < nk ? s : nk - 1; };
+  v4i BR[4][2], BI[4][2];
+  issue_rows(0, 0);
+  __builtin_amdgcn_sched_barrier(0);
+  issue_rows(cl(1), 1);
+  load_cols(BR[0][0], BR[0][1], BI[0][0], BI[0][1], 0);
+  __builtin_amdgcn_sched_barrier(0);
+  issue_rows(cl(2), 2);
+  load_cols(BR[1][0], BR[1][1], BI[1][0], BI[1][1], cl(1));
+  __builtin_amdgcn_sched_barrier(0);
+  issue_rows(cl(3), 3);
+  load_cols(BR[2][0], BR[2][1], BI[2][0], BI[2][1], cl(2));
+  __builtin_amdgcn_sched_barrier(0);
   asm volatile("s_waitcnt vmcnt(18)\n\ts_barrier" ::: "memory");
-  read_a(ar, ai, 0);
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
-    br[j] = load_op(bB, 0, 0, j);
-    bi[j] = load_op(bB, 0, 1, j);
+  for (int i = 0; i < 3; i++) {
+    ar[i] = load_op(0, 0, i);
+    ai[i] = load_op(0, 1, i);
   }
+  ar3n = load_op(0, 0, 3);
+  ai3n = load_op(0, 1, 3);
 
   // K-step ks (slot ks % kRing):
-  //   wait for this wave's loads of ks+1 (those of ks+2, ks+3 may stay
-  //   pending), barrier: every wave's loads of ks+1 landed; every wave's
-  //   reads of slot ks are done (they were waited for before the barrier);
-  //   refill slot ks with k-step ks+kRing; MFMAs of ks, reading ks+1's
-  //   operands as ks's die.
+  //   wait until G(ks-3) landed (this wave's DMAs of ks+1 and column loads
+  //   of ks; G(ks-2), G(ks-1) may stay pending), barrier: every wave's DMAs
+  //   of ks+1 landed and every wave's reads of slot ks are done; refill
+  //   slot ks with k-step ks+4, load the columns of ks+3; MFMAs of ks,
+  //   reading ks+1's row operands as ks's die.
   for (int k00 = 0; k00 < nk; k00 += kRing) {
 #pragma unroll
     for (int d = 0; d < kRing; d++) {
-      const int ks = k00 + d;
-      const int nx = ks + kRing;
+      const int ks = k00 + d, n3 = (d + 3) % kRing;
       __builtin_amdgcn_sched_barrier(0);
       asm volatile("s_waitcnt vmcnt(12)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
       __builtin_amdgcn_sched_barrier(0);
-      step_ops((d + 1) % kRing);
-      issue(nx < nk ? nx : nk - 1, d);
-      // group 0: 8 MFMAs with the 4 reads of the next row operands
+      issue_rows(cl(ks + 4), d);
+      load_cols(BR[n3][0], BR[n3][1], BI[n3][0], BI[n3][1], cl(ks + 3));
+      step_ops(BR[d][0], BR[d][1], BI[d][0], BI[d][1], (d + 1) % kRing);
+      // 8 MFMAs per row group i; VMEM ops (2 DMAs, 4 column loads) over
+      // groups 0-1, LDS reads of the next row operands from group 1 on
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
-        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
+      for (int i = 0; i < 8; i++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (i < 3) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
       }
-      __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
-      // groups 1-3: 2 reads (the previous group's next operands) and 2 DMAs
-      // over each group's 8 MFMAs
 #pragma unroll
-      for (int g = 0; g < 3; g++) {
-#pragma unroll
-        for (int i = 0; i < 2; i++) {
-          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-        }
-#pragma unroll
-        for (int i = 0; i < 2; i++) {
-          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-          __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM (LDS DMA)
-        }
-        __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+      for (int i = 0; i < 8; i++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (i < 2) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        if (i >= 4 && i < 7) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
       }
-      __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // the last group's next operands
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (i < 4) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      }
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (i < 2) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      }
       __builtin_amdgcn_sched_barrier(0);
     }
   }
   // Every partial sum is a multiple of 64 of magnitude <= 2 * 64 * 64 *
   // 32 * (2 * kMaxItemStages) = 2^30: no int32 overflow.
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < 4; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < 2; j++) {
       const int by = by0 + i, bx = bx0 + j;
       if (bx >= by && bx < nb) {
         const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
@@ -422,8 +453,11 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(GemmJob J, int nst, i
 __global__ __launch_bounds__(256, 1) void corr_mfma_batch_kernel(
     const GemmJob *__restrict__ jobs, const GemmItem *__restrict__ items,
     const int *__restrict__ qbound, int *__restrict__ counters) {
-  __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kSlotBytes];
-  __shared__ int next;
+  // One LDS object: the ring and, past it, the item handed to the
+  // workgroup.  (A second __shared__ variable makes the compiler wait for
+  // every buffer->LDS load before each LDS read in the loop.)
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing + 1][kSlotBytes];
+  int &next = *(int *)&lds[kRing][0];
   int q = blockIdx.x % 8, moves = 0;
   for (;;) {
     if (threadIdx.x == 0) {
