@@ -6,8 +6,8 @@
 // both operands E3M2 (and block scales 2^0) issues a 32x32x64 product in the
 // cycles an int8 32x32x32 one takes, and accumulates in FP32.  Per 32x32
 // block two accumulators:
-//   RE = Ar.Br^T + Ai.Bi^T,  IM = Ar.Bi^T + (-Ai).Br^T,  im = U ? -IM : IM,
-// where -Ai is Ai with the sign bit of every code flipped (exact).  Every
+//   RE = Ar.Br^T + Ai.Bi^T,  IM = Ar.Bi^T + Ai.(-Br)^T,  im = U ? -IM : IM,
+// where -Br is Br with the sign bit of every code flipped (exact).  Every
 // product is an integer of magnitude <= 64, so a sample adds at most 128 to
 // an accumulator; a work item covers at most kMaxItemStages = 2048 stages of
 // 64 samples, so every partial sum is an integer of magnitude <= 2^24 and
@@ -369,27 +369,40 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
 #pragma unroll
     for (int j = 0; j < 2; j++) re[i][j] = im[i][j] = zero;
   const uint8_t *lds0 = &lds[0][0];
-  const int bA0 = lane * 16, bA1 = kP0Bytes + lane * 8;
+  // LDS read addresses.  The 8-byte P1 reads of the eight row operands get
+  // one opaque base register each, so that the compiler does not merge two
+  // of them into one read2 (whose halves it would then have to move next
+  // to their P0 parts).
+  const int bA0 = lane * 16;
+  int bA1[2][kRowGroups];
+#pragma unroll
+  for (int pl = 0; pl < 2; pl++)
+#pragma unroll
+    for (int k = 0; k < kRowGroups; k++)
+      asm volatile("v_mov_b32 %0, %1"
+                   : "=v"(bA1[pl][k])
+                   : "v"(kP0Bytes + lane * 8 + pl * kRowGroups * 512 + k * 512));
   auto load_op = [&](int slot, int plane, int k) -> v8i {
     const v4i a = *(const v4i *)(lds0 + bA0 + (slot * kSlotBytes + plane * kRowGroups * 1024 + k * 1024));
-    const v2i b = *(const v2i *)(lds0 + bA1 + (slot * kSlotBytes + plane * kRowGroups * 512 + k * 512));
+    const v2i b = *(const v2i *)(lds0 + bA1[plane][k] + slot * kSlotBytes);
     return (v8i){a.x, a.y, a.z, a.w, b.x, b.y, 0, 0};
   };
   const v8i sign = {0x20820820, 0x08208208, (int)0x82082082u, 0x20820820, 0x08208208,
                     (int)0x82082082u, 0, 0};
-  // Row operands of the current stage (row group 3 double-buffered: its
-  // next operands are read before its MFMAs end).
-  v8i ar[4], ai[4], ar3n, ai3n;
-  // One stage with column operands (br0, br1, bi0, bi1): for each row group
-  // i its 8 MFMAs, after which that group's operands of the next stage
-  // (slot nslot) replace it.
+  // Row operands: two sets, alternating by stage parity: a stage computes
+  // with one and reads the next stage's into the other.
+  v8i AR[2][4], AI[2][4];
+  // One stage with column operands (br0, br1, bi0, bi1) and row set p:
+  // RE += Ar.Br + Ai.Bi, IM += Ar.Bi + Ai.(-Br), -Br being Br with the
+  // sign bit of every code flipped.  The next stage's row operands (slot
+  // nslot) are read into set p ^ 1.
   auto stage_ops = [&](const v8i &br0, const v8i &br1, const v8i &bi0, const v8i &bi1,
-                       int nslot) {
+                       int nslot, int p) {
     const v8i br[2] = {br0, br1}, bi[2] = {bi0, bi1};
+    const v8i nbr[2] = {br0 ^ sign, br1 ^ sign};
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const v8i xr = i == 3 ? ar3n : ar[i], xi = i == 3 ? ai3n : ai[i];
-      const v8i nxi = xi ^ sign;
+      const v8i xr = AR[p][i], xi = AI[p][i];
 #pragma unroll
       for (int j = 0; j < 2; j++) {
         re[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(xr, br[j], re[i][j], 3, 3,
@@ -401,20 +414,12 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
       for (int j = 0; j < 2; j++) {
         re[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(xi, bi[j], re[i][j], 3, 3,
                                                                    0, 127, 0, 127);
-        im[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(nxi, br[j], im[i][j], 3, 3,
+        im[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(xi, nbr[j], im[i][j], 3, 3,
                                                                    0, 127, 0, 127);
       }
-      if (i < 3) {
-        ar[i] = load_op(nslot, 0, i);
-        ai[i] = load_op(nslot, 1, i);
-      }
-      if (i == 1) {  // the next stage's row group 3, into the second buffer
-        ar[3] = load_op(nslot, 0, 3);
-        ai[3] = load_op(nslot, 1, 3);
-      }
+      AR[p ^ 1][i] = load_op(nslot, 0, i);
+      AI[p ^ 1][i] = load_op(nslot, 1, i);
     }
-    ar3n = ar[3];
-    ai3n = ai[3];
   };
   const int by0 = ti * kTile / 32, bx0 = (ct * kTile + 64 * wave) / 32;
 
@@ -441,19 +446,17 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   __builtin_amdgcn_sched_barrier(0);
   asm volatile("s_waitcnt vmcnt(33)\n\ts_barrier" ::: "memory");
 #pragma unroll
-  for (int i = 0; i < 3; i++) {
-    ar[i] = load_op(0, 0, i);
-    ai[i] = load_op(0, 1, i);
+  for (int i = 0; i < 4; i++) {
+    AR[0][i] = load_op(0, 0, i);
+    AI[0][i] = load_op(0, 1, i);
   }
-  ar3n = load_op(0, 0, 3);
-  ai3n = load_op(0, 1, 3);
 
   // Stage st (slot st % kRing):
   //   wait until G(st-3) landed (this wave's DMAs of st+1 and column loads
   //   of st; G(st-2), G(st-1) may stay pending), barrier: every wave's DMAs
   //   of st+1 landed and every wave's reads of slot st are done; refill
   //   slot st with stage st+4, load the columns of st+3; MFMAs of st,
-  //   reading st+1's row operands as st's die.
+  //   reading st+1's row operands into the other row set meanwhile.
   for (int st0 = 0; st0 < nst; st0 += kRing) {
 #pragma unroll
     for (int d = 0; d < kRing; d++) {
@@ -463,31 +466,18 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
       __builtin_amdgcn_sched_barrier(0);
       issue_rows(cl(st + 4), d);
       load_cols(BR[n3][0], BR[n3][1], BI[n3][0], BI[n3][1], cl(st + 3));
-      stage_ops(BR[d][0], BR[d][1], BI[d][0], BI[d][1], (d + 1) % kRing);
-      // 8 MFMAs per row group i; VMEM ops (3 DMAs, 8 column loads) over
-      // groups 0-2, LDS reads of the next row operands from group 1 on
+      stage_ops(BR[d][0], BR[d][1], BI[d][0], BI[d][1], (d + 1) % kRing, d & 1);
+      // 8 MFMAs per row group i, each group with 4 LDS reads of the next
+      // stage's row operands; VMEM ops (3 DMAs, 8 column loads) over
+      // groups 0-2
 #pragma unroll
-      for (int i = 0; i < 8; i++) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-        if (i < 4) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
-      }
+      for (int g = 0; g < 4; g++)
 #pragma unroll
-      for (int i = 0; i < 8; i++) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-        if (i < 4) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-        if (i >= 4) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
-      }
-#pragma unroll
-      for (int i = 0; i < 8; i++) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-        if (i < 8) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-        if (i < 3) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
-      }
-#pragma unroll
-      for (int i = 0; i < 8; i++) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-        if (i < 4) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-      }
+        for (int i = 0; i < 8; i++) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+          if (i % 2 == 1) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+          if (g < 3 && i < (g == 2 ? 3 : 4)) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+        }
       __builtin_amdgcn_sched_barrier(0);
     }
   }

@@ -19,8 +19,8 @@
 //     and samples past T are 0.
 //  2. corr_mfma_kernel: a workgroup = 4 wavefronts = a 128-row x 256-column
 //     tile of one channel, each wave all 128 rows x 64 columns (4x2 blocks
-//     x 2 int32 accumulators: RE = Ar.Br + Ai.Bi and IM = Ar.Bi + (-Ai).Br,
-//     -Ai negated per byte in registers).  The row operands come through a
+//     x 2 int32 accumulators: RE = Ar.Br + Ai.Bi and IM = Ar.Bi + Ai.(-Br),
+//     -Br negated per byte in registers).  The row operands come through a
 //     ring of 4 LDS k-steps of 32 samples (8 KB each) filled by buffer->LDS
 //     DMA; each wave loads its own column operands straight into VGPRs,
 //     three k-steps ahead.
@@ -220,7 +220,7 @@ __device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti, int &ct
 // channel f over 64-sample stages [st_begin, st_begin + nst) (nst a multiple
 // of kMfmaRing), run as 32-sample k-steps.  Wave w computes all 128 rows x
 // columns 64 w.. as 4x2 blocks of two int32 accumulators (256 AGPRs):
-// RE += Ar.Br + Ai.Bi,  IM += Ar.Bi + (-Ai).Br,  where -Ai is negated per
+// RE += Ar.Br + Ai.Bi,  IM += Ar.Bi + Ai.(-Br),  where -Br is negated per
 // byte in registers (exact: every byte is 8 x value, in [-64, 56]).  The
 // row operands, shared by the four waves, come through the LDS ring by
 // buffer->LDS DMA; each wave loads its own column operands straight into
@@ -304,11 +304,12 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
   auto step_ops = [&](const v4i &br0, const v4i &br1, const v4i &bi0, const v4i &bi1,
                       int nslot) {
     const v4i br[2] = {br0, br1}, bi[2] = {bi0, bi1};
+    // -Br per byte (bytes are multiples of 8): IM += Ar.Bi + Ai.(-Br)
+    const v4i nbr[2] = {((br0 ^ (int)0xF8F8F8F8u) + 0x08080808) & (int)0xF8F8F8F8u,
+                        ((br1 ^ (int)0xF8F8F8F8u) + 0x08080808) & (int)0xF8F8F8F8u};
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const v4i xr = i == 3 ? ar3n : ar[i], xi = i == 3 ? ai3n : ai[i];
-      // -x per byte: bytes are multiples of 8
-      const v4i nxi = ((xi ^ (int)0xF8F8F8F8u) + 0x08080808) & (int)0xF8F8F8F8u;
 #pragma unroll
       for (int j = 0; j < 2; j++) {
         re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(xr, br[j], re[i][j], 0, 0, 0);
@@ -317,7 +318,7 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
 #pragma unroll
       for (int j = 0; j < 2; j++) {
         re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(xi, bi[j], re[i][j], 0, 0, 0);
-        im[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(nxi, br[j], im[i][j], 0, 0, 0);
+        im[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(xi, nbr[j], im[i][j], 0, 0, 0);
       }
       if (i < 3) {
         ar[i] = load_op(nslot, 0, i);
