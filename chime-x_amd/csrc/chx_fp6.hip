@@ -126,7 +126,9 @@ __device__ __forceinline__ void unpack_f6_unit(const uint8_t *__restrict__ in,
 #pragma unroll
     for (int s = 0; s < 32; s++) {
       const bool ok = evalid && (ts + s < t_valid);
-      w[s] = ok ? *(const uint32_t *)(src + s * row_bytes) : 0x88888888u;  // 0x88 -> code 0
+      // streamed once: non-temporal loads (and stores below)
+      w[s] = ok ? __builtin_nontemporal_load((const uint32_t *)(src + s * row_bytes))
+                : 0x88888888u;  // 0x88 -> code 0
     }
 #pragma unroll
     for (int m = 0; m < 8; m++)
@@ -164,8 +166,8 @@ __device__ __forceinline__ void unpack_f6_unit(const uint8_t *__restrict__ in,
       const int el = k * 256 + tid;
       if (el < n_el) {
         const int slot = el + (el >> 4);
-        *(v4u *)(d0 + el * 16) = s0[slot];
-        *(v2u *)(d1 + el * 8) = s1[slot];
+        __builtin_nontemporal_store(s0[slot], (v4u *)(d0 + el * 16));
+        __builtin_nontemporal_store(s1[slot], (v2u *)(d1 + el * 8));
       }
     }
     __syncthreads();

@@ -133,7 +133,8 @@ __device__ __forceinline__ void unpack_unit(const uint8_t *__restrict__ in,
 #pragma unroll
   for (int s = 0; s < 16; s++) {
     const bool ok = evalid && (16 * ks + s < t_valid);
-    w[s] = ok ? *(const uint32_t *)(src + s * row_bytes) : 0x88888888u;
+    // streamed once: non-temporal loads (and stores below)
+    w[s] = ok ? __builtin_nontemporal_load((const uint32_t *)(src + s * row_bytes)) : 0x88888888u;
   }
   uint32_t c[4][4];  // [element][4-sample group]
 #pragma unroll
@@ -160,8 +161,8 @@ __device__ __forceinline__ void unpack_unit(const uint8_t *__restrict__ in,
     const int el = k * 256 + tid;
     if (el < n_el) {
       const int slot = el + (el >> 4);
-      *(v4u *)(dst + el * 16) = stage[0][slot];
-      *(v4u *)(dst + plane_stride + el * 16) = stage[1][slot];
+      __builtin_nontemporal_store(stage[0][slot], (v4u *)(dst + el * 16));
+      __builtin_nontemporal_store(stage[1][slot], (v4u *)(dst + plane_stride + el * 16));
     }
   }
 }
