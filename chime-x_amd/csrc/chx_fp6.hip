@@ -52,7 +52,8 @@ constexpr int kMaxStages = kMaxItemStages;  // stages of one work item
 constexpr int kRowGroups = 4;        // 32-element row groups per stage (in LDS)
 constexpr int kP0Bytes = 2 * kRowGroups * 1024;  // [plane 2][group 4][64 lanes][16 B]
 constexpr int kP1Bytes = 2 * kRowGroups * 512;   // [plane 2][group 4][64 lanes][8 B]
-constexpr int kSlotBytes = kP0Bytes + kP1Bytes;
+constexpr int kSlotBytes = kP0Bytes + kP1Bytes;  // one 128-row block of one stage
+constexpr int kRingBytes = kRing * kSlotBytes;   // a ring of one row block
 static_assert(kMaxStages % kRing == 0, "work items hold whole rings");
 
 // Four nibbles (one per byte, 0-15) -> the E3M2 codes of nibble - 8 (one per
@@ -260,30 +261,44 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t id, int64_t total) {
   return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
 }
 
-// Tile tb of one channel: rows [kTile ti, +kTile), columns [kTile ct,
-// +2 kTile), row-major over ti, then ct = ti, ti + 2, ...  Row ti holds
-// ceil((ntiles - ti) / 2) tiles; the last one's second column half lies past
-// the array when ntiles - ti is odd.
-__device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti, int &ct) {
+// Tile tb of one channel.  First the full 128 x 256 tiles, row by row: row
+// tile ti holds floor((ntiles - ti) / 2) of them, columns from kTile ct, ct
+// = ti, ti + 2, ...  Then the half tiles, the last tile of each row tile ti
+// with ntiles - ti odd (ti = ntiles-1, ntiles-3, ...), whose 128 columns
+// are column tile ntiles-1: two of them form one item (rows ti_a for waves
+// 0-1, ti_b = ti_a - 2 for waves 2-3); an odd last one stays alone (ti_b <
+// 0; its waves 2-3 compute past the array).  Full tiles return ti_b = ti_a.
+__device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti_a, int &ti_b,
+                                            int &ct) {
   int r = 0;
-  while (tb >= (ntiles - r + 1) / 2) {
-    tb -= (ntiles - r + 1) / 2;
+  while (r < ntiles && tb >= (ntiles - r) / 2) {
+    tb -= (ntiles - r) / 2;
     r++;
   }
-  ti = r;
-  ct = r + 2 * tb;
+  if (r < ntiles) {
+    ti_a = ti_b = r;
+    ct = r + 2 * tb;
+  } else {
+    ct = ntiles - 1;
+    ti_a = ntiles - 1 - 4 * tb;
+    ti_b = ti_a - 2;
+  }
 }
 
-// Pass 2, one output tile (rows from kTile ti, columns from kTile ct) of
-// channel f over k-steps [st_begin, st_begin + nst) (nst a multiple of
-// kRing).  Wave w computes all 128 rows x columns 64 w.. as 4x2 blocks of
-// two FP32 accumulators (256 AGPRs).  Row operands (4 groups x 2 planes,
-// shared by the four waves) go through the LDS ring by buffer->LDS DMA;
-// each wave loads its own column operands (2 groups x 2 planes) straight
-// into VGPRs, three stages ahead.
-__device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, const int ti,
-                                             const int ct, const int st_begin, const int nst,
-                                             uint8_t (*lds)[kSlotBytes]) {
+// Pass 2, one output item of channel f over k-steps [st_begin, st_begin +
+// nst) (nst a multiple of kRing).  Wave w computes 128 rows x 64 columns as
+// 4x2 blocks of two FP32 accumulators (256 AGPRs):
+//   !PAIR: rows from kTile ti_a, columns from kTile ct + 64 w (a 128 x 256
+//          tile; the four waves share the rows);
+//    PAIR: two half tiles of column tile ct: waves 0-1 rows from kTile
+//          ti_a, waves 2-3 rows from kTile ti_b, columns kTile ct + 64 (w%2).
+// Row operands (4 groups x 2 planes per 128-row block) go through the LDS
+// ring by buffer->LDS DMA; each wave loads its own column operands (2
+// groups x 2 planes) straight into VGPRs, three stages ahead.
+template <bool PAIR>
+__device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, const int ti_a,
+                                             const int ti_b, const int ct, const int st_begin,
+                                             const int nst, uint8_t (*lds)[kSlotBytes]) {
   const int F = J.F, Np = J.Np, KB = J.KB, nb = J.nb, U = J.U, mode = J.mode;
   const int64_t NB = J.NB;
   int32_t *vis = J.vis;
@@ -302,36 +317,52 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
         (void *)(J.planes1 + ((int64_t)pl * F + f) * KB * p1_blk), (short)0, (int)(KB * p1_blk),
         0x00020000);
   }
-  // Row DMAs: wave w fills plane w/2, row groups 2(w&1), 2(w&1)+1: two P0
-  // operands and one P1 group pair per stage.  Row group g holds elements
-  // kTile ti + 32 g (always inside the array).
-  const int lp = wave >> 1, gh = wave & 1;
-  int voff0[2], voff1;
+  // Row DMAs.  !PAIR: wave w fills plane w/2, row groups 2(w&1), 2(w&1)+1
+  // of the one row block: two P0 operands and one P1 group pair per stage.
+  // PAIR: wave w fills plane w&1, all four row groups of row block w/2
+  // (rows ti_a, ti_b; ring w/2 of two): four P0 operands
+  // and two P1 group pairs.  Row group g holds elements kTile ti + 32 g
+  // (always inside the array).
+  constexpr int kDma = PAIR ? 6 : 3;  // row DMAs per wave and stage
+  constexpr int kP0n = PAIR ? 4 : 2, kP1n = PAIR ? 2 : 1;
+  const int rb = PAIR ? wave >> 1 : 0;               // this wave's row block
+  const int lp = PAIR ? wave & 1 : wave >> 1;        // plane it fills
+  const int g0 = PAIR ? 0 : 2 * (wave & 1);          // first row group it fills
+  const int ti = PAIR && rb ? ti_b : ti_a;           // this wave's row tile
+  int voff0[kP0n], voff1[kP1n];
 #pragma unroll
-  for (int gg = 0; gg < 2; gg++)
-    voff0[gg] = ((lane >> 5) * Np + ti * kTile + 32 * (2 * gh + gg) + (lane & 31)) * 16;
-  {
-    const int g = 2 * gh + (lane >> 5);  // lanes 32-63: the pair's second group
+  for (int gg = 0; gg < kP0n; gg++)
+    voff0[gg] = ((lane >> 5) * Np + ti * kTile + 32 * (g0 + gg) + (lane & 31)) * 16;
+#pragma unroll
+  for (int pp = 0; pp < kP1n; pp++) {
+    const int g = g0 + 2 * pp + (lane >> 5);  // lanes 32-63: the pair's second group
     const int i = lane & 31;
-    voff1 = ((i >> 4) * Np + ti * kTile + 32 * g + 2 * (i & 15)) * 8;
+    voff1[pp] = ((i >> 4) * Np + ti * kTile + 32 * g + 2 * (i & 15)) * 8;
   }
   auto issue_rows = [&](int st, int slot) {
     const int kb = st_begin + st;
 #pragma unroll
-    for (int gg = 0; gg < 2; gg++) {
-      lds_void *dst = (lds_void *)&lds[slot][(lp * kRowGroups + 2 * gh + gg) * 1024];
+    for (int gg = 0; gg < kP0n; gg++) {
+      lds_void *dst =
+          (lds_void *)&lds[rb * kRing + slot][(lp * kRowGroups + g0 + gg) * 1024];
       __builtin_amdgcn_raw_ptr_buffer_load_lds(r0[lp], dst, 16, voff0[gg], kb * (int)p0_blk, 0,
                                                0);
     }
-    lds_void *dst = (lds_void *)&lds[slot][kP0Bytes + (lp * kRowGroups + 2 * gh) * 512];
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(r1[lp], dst, 16, voff1, kb * (int)p1_blk, 0, 0);
+#pragma unroll
+    for (int pp = 0; pp < kP1n; pp++) {
+      lds_void *dst =
+          (lds_void *)&lds[rb * kRing + slot][kP0Bytes + (lp * kRowGroups + g0 + 2 * pp) * 512];
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(r1[lp], dst, 16, voff1[pp], kb * (int)p1_blk, 0,
+                                               0);
+    }
   };
   // Column operands of this wave: groups 2 w + j, lane -> (half lane/32,
   // element + lane%32).  Columns past the array (the right half of a row's
   // last tile when the tile count is odd: waves 2 and 3) load through an
   // empty descriptor, which fetches nothing and returns zeros; their
   // products are never stored.
-  const bool past = wave >= 2 && (ct + 1) * kTile >= Np;
+  const bool past = !PAIR && wave >= 2 && (ct + 1) * kTile >= Np;
+  const int cw = PAIR ? wave & 1 : wave;  // column quarter of the tile
   const __amdgpu_buffer_rsrc_t rz = __builtin_amdgcn_make_buffer_rsrc(
       (void *)J.planes, (short)0, 0, 0x00020000);
   __amdgpu_buffer_rsrc_t c0[2], c1[2];
@@ -343,7 +374,7 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   int boff0[2], boff1[2];
 #pragma unroll
   for (int j = 0; j < 2; j++) {
-    int e = ct * kTile + 64 * wave + 32 * j;
+    int e = ct * kTile + 64 * cw + 32 * j;
     if (e >= Np) e -= kTile;
     e += lane & 31;
     boff0[j] = ((lane >> 5) * Np + e) * 16;
@@ -375,7 +406,7 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   // one opaque base register each, so that the compiler does not merge two
   // of them into one read2 (whose halves it would then have to move next
   // to their P0 parts).
-  const int bA0 = lane * 16;
+  const int bA0 = rb * kRingBytes + lane * 16;
   int bA1[2][kRowGroups];
 #pragma unroll
   for (int pl = 0; pl < 2; pl++)
@@ -383,7 +414,8 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
     for (int k = 0; k < kRowGroups; k++)
       asm volatile("v_mov_b32 %0, %1"
                    : "=v"(bA1[pl][k])
-                   : "v"(kP0Bytes + lane * 8 + pl * kRowGroups * 512 + k * 512));
+                   : "v"(rb * kRingBytes + kP0Bytes + lane * 8 + pl * kRowGroups * 512 +
+                         k * 512));
   auto load_op = [&](int slot, int plane, int k) -> v8i {
     const v4i a = *(const v4i *)(lds0 + bA0 + (slot * kSlotBytes + plane * kRowGroups * 1024 + k * 1024));
     const v2i b = *(const v2i *)(lds0 + bA1[plane][k] + slot * kSlotBytes);
@@ -423,13 +455,13 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
       AI[p ^ 1][i] = load_op(nslot, 1, i);
     }
   };
-  const int by0 = ti * kTile / 32, bx0 = (ct * kTile + 64 * wave) / 32;
+  const int by0 = ti * kTile / 32, bx0 = (ct * kTile + 64 * cw) / 32;
 
   // Issue groups: G(st), issued during stage st, = the row DMAs of stage
-  // st+4 (3) and the column loads of stage st+3 (8).  Prologue: G(-4) =
+  // st+4 (kDma) and the column loads of stage st+3 (8).  Prologue: G(-4) =
   // DMAs of stage 0, G(-3..-1) = DMAs of stages 1..3 and column loads of
   // 0..2 (sched barriers keep the issue order the waits count on).  Wait
-  // for this wave's row DMAs of stage 0 (33 later operations may stay
+  // for this wave's row DMAs of stage 0 (3 |G| later operations may stay
   // pending), barrier, read them.  Column operands of stage s live in set
   // s % 4: stage st loads st+3 into the set stage st-1 has finished.
   static_assert(kRing == 4, "column loads three stages ahead, row DMAs four");
@@ -446,7 +478,10 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   issue_rows(cl(3), 3);
   load_cols(BR[2][0], BR[2][1], BI[2][0], BI[2][1], cl(2));
   __builtin_amdgcn_sched_barrier(0);
-  asm volatile("s_waitcnt vmcnt(33)\n\ts_barrier" ::: "memory");
+  if constexpr (PAIR)
+    asm volatile("s_waitcnt vmcnt(42)\n\ts_barrier" ::: "memory");
+  else
+    asm volatile("s_waitcnt vmcnt(33)\n\ts_barrier" ::: "memory");
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     AR[0][i] = load_op(0, 0, i);
@@ -455,7 +490,7 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
 
   // Stage st (slot st % kRing):
   //   wait until G(st-3) landed (this wave's DMAs of st+1 and column loads
-  //   of st; G(st-2), G(st-1) may stay pending), barrier: every wave's DMAs
+  //   of st; G(st-2), G(st-1), 2 |G| operations, may stay pending), barrier: every wave's DMAs
   //   of st+1 landed and every wave's reads of slot st are done; refill
   //   slot st with stage st+4, load the columns of st+3; MFMAs of st,
   //   reading st+1's row operands into the other row set meanwhile.
@@ -464,13 +499,16 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
     for (int d = 0; d < kRing; d++) {
       const int st = st0 + d, n3 = (d + 3) % kRing;
       __builtin_amdgcn_sched_barrier(0);
-      asm volatile("s_waitcnt vmcnt(22)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+      if constexpr (PAIR)
+        asm volatile("s_waitcnt vmcnt(28)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+      else
+        asm volatile("s_waitcnt vmcnt(22)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
       __builtin_amdgcn_sched_barrier(0);
       issue_rows(cl(st + 4), d);
       load_cols(BR[n3][0], BR[n3][1], BI[n3][0], BI[n3][1], cl(st + 3));
       stage_ops(BR[d][0], BR[d][1], BI[d][0], BI[d][1], (d + 1) % kRing, d & 1);
       // 8 MFMAs per row group i, each group with 4 LDS reads of the next
-      // stage's row operands; VMEM ops (3 DMAs, 8 column loads) over
+      // stage's row operands; VMEM ops (kDma DMAs, 8 column loads) over
       // groups 0-2
 #pragma unroll
       for (int g = 0; g < 4; g++)
@@ -478,7 +516,8 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
         for (int i = 0; i < 8; i++) {
           __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
           if (i % 2 == 1) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-          if (g < 3 && i < (g == 2 ? 3 : 4)) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+          if (g < 3 && i < (PAIR ? (g == 2 ? 4 : 5) : (g == 2 ? 3 : 4)))
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
         }
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -504,8 +543,8 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
 // blockIdx % 8 = XCD).
 __global__ __launch_bounds__(256, 1) void corr_f6_kernel(GemmJob J, int head, int tail,
                                                          int tmode) {
-  __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kSlotBytes];
-  const int64_t per_f = wide_tiles(J.ntiles);
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * kRing][kSlotBytes];  // two rings
+  const int64_t per_f = fp6_tiles(J.ntiles);
   int64_t item;
   int s0, ns;
   if ((int)blockIdx.x < head) {
@@ -520,9 +559,12 @@ __global__ __launch_bounds__(256, 1) void corr_f6_kernel(GemmJob J, int head, in
     ns = J.KB - s0 < J.stages_per_split ? J.KB - s0 : J.stages_per_split;
     J.mode = tmode;
   }
-  int ti, ct;
-  tile_coords((int)(item % per_f), J.ntiles, ti, ct);
-  gemm_tile_f6(J, (int)(item / per_f), ti, ct, s0, ns, lds);
+  int ti_a, ti_b, ct;
+  tile_coords((int)(item % per_f), J.ntiles, ti_a, ti_b, ct);
+  if (ti_b >= 0 && ti_b != ti_a)
+    gemm_tile_f6<true>(J, (int)(item / per_f), ti_a, ti_b, ct, s0, ns, lds);
+  else
+    gemm_tile_f6<false>(J, (int)(item / per_f), ti_a, ti_a, ct, s0, ns, lds);
 }
 
 // Batch: persistent workgroups take work items from 8 per-XCD queues (the
@@ -534,8 +576,8 @@ __global__ __launch_bounds__(256, 1) void corr_f6_batch_kernel(
   // One LDS object: the ring and, past it, the item handed to the
   // workgroup.  (A second __shared__ variable makes the compiler wait for
   // every buffer->LDS load before each LDS read in the loop.)
-  __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing + 1][kSlotBytes];
-  int &next = *(int *)&lds[kRing][0];
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * kRing + 1][kSlotBytes];
+  int &next = *(int *)&lds[2 * kRing][0];
   int q = blockIdx.x % 8, moves = 0;
   for (;;) {
     if (threadIdx.x == 0) {
@@ -557,9 +599,12 @@ __global__ __launch_bounds__(256, 1) void corr_f6_batch_kernel(
     if (it < 0) break;
     const GemmItem w = items[it];
     const GemmJob J = jobs[w.job];
-    int ti, ct;
-    tile_coords(w.tile, J.ntiles, ti, ct);
-    gemm_tile_f6(J, w.f, ti, ct, w.stage0, w.nstages, lds);
+    int ti_a, ti_b, ct;
+    tile_coords(w.tile, J.ntiles, ti_a, ti_b, ct);
+    if (ti_b >= 0 && ti_b != ti_a)
+      gemm_tile_f6<true>(J, w.f, ti_a, ti_b, ct, w.stage0, w.nstages, lds);
+    else
+      gemm_tile_f6<false>(J, w.f, ti_a, ti_a, ct, w.stage0, w.nstages, lds);
   }
 }
 
@@ -610,7 +655,7 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
   const int N = p->cfg.num_elements, F = p->cfg.num_freq, U = p->cfg.upper_triangle_convention;
   const int64_t T = p->cfg.time_steps;
   const int ntiles = p->ntiles, Np = ntiles * kTile;
-  const int64_t items = (int64_t)F * wide_tiles(ntiles);
+  const int64_t items = (int64_t)F * fp6_tiles(ntiles);
   if (items > 0x7FFFFFFF || !p->d_work) return CHX_ERR_SHAPE;
   int cus = 0;
   CHX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->cfg.device));
@@ -650,7 +695,7 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
     const int tmode = split > 1 ? 2 : kmode;  // the tail's pieces add atomically
     if (tail > 0 && split > 1 && kmode == 0) {
       // zero the channels the tail touches (before the head's stores)
-      const int64_t f0 = head / wide_tiles(ntiles);
+      const int64_t f0 = head / fp6_tiles(ntiles);
       int rc = launch_fill_u32((uint32_t *)vis + f0 * p->NB * 2048, 0u,
                                (int64_t)(F - f0) * p->NB * 2048, s);
       if (rc) return rc;

@@ -161,9 +161,28 @@ def _fake(jobs):
 
 
 def _tiles(nt, method):
-    """Output tiles of one channel (both matrix-core methods): 128 x 256, row
-    tile i holding ceil((nt - i) / 2) of them."""
-    return sum((nt - i + 1) // 2 for i in range(nt))
+    """Work items of one channel: tiles of 128 x 256, row tile i holding
+    ceil((nt - i) / 2) of them; FP6 (method 3) puts the half tiles (the last
+    one of each row tile with nt - i odd) two to an item."""
+    wide = sum((nt - i + 1) // 2 for i in range(nt))
+    return wide - (nt + 1) // 2 // 2 if method == 3 else wide
+
+
+def _fp6_item_blocks(nt):
+    """The 32x32 blocks each FP6 work item of one channel computes, following
+    the kernel's item order: full tiles row by row, then pairs of half tiles
+    (rows nt-1-4q and nt-3-4q, column tile nt-1), an odd last one alone."""
+    items = []
+    for r in range(nt):
+        for k in range((nt - r) // 2):
+            ct = r + 2 * k
+            items.append({(4 * r + i, 4 * ct + j) for i in range(4) for j in range(8)})
+    halves = list(range(nt - 1, -1, -2))
+    for q in range(0, len(halves), 2):
+        rows = halves[q:q + 2]
+        items.append({(4 * r + i, 4 * (nt - 1) + j) for r in rows for i in range(4)
+                      for j in range(4)})
+    return items
 
 
 def _expected_items(jobs, chunk, max_st, unit, method):
@@ -183,10 +202,31 @@ def _expected_items(jobs, chunk, max_st, unit, method):
     return rounds, n
 
 
-def test_fp6_tile_count_closed_form():
+def test_tile_count_closed_forms():
     for nt in range(1, 40):
         m = nt // 2
-        assert _tiles(nt, 3) == ((m + 1) ** 2 if nt % 2 else m * (m + 1))
+        wide = (m + 1) ** 2 if nt % 2 else m * (m + 1)
+        assert _tiles(nt, 2) == wide
+        assert _tiles(nt, 3) == wide - (nt + 1) // 2 // 2
+
+
+def test_fp6_items_cover_the_upper_triangle_once():
+    # Every upper-triangle 32x32 block of an nt-tile array is stored by
+    # exactly one FP6 work item (blocks below the diagonal are computed and
+    # not stored; a lone half tile's columns past the array are not listed,
+    # as they are never stored), and the item count is the planner's.
+    for nt in range(1, 24):
+        items = _fp6_item_blocks(nt)
+        assert len(items) == _tiles(nt, 3)
+        seen = {}
+        for k, blocks in enumerate(items):
+            for (by, bx) in blocks:
+                if bx >= by and bx < 4 * nt:
+                    assert (by, bx) not in seen, (nt, by, bx)
+                    seen[(by, bx)] = k
+        assert len(seen) == 4 * nt * (4 * nt + 1) // 2
+        for blocks in items:
+            assert all(bx < 4 * nt for (_, bx) in blocks)
 
 
 @pytest.mark.parametrize("method", [2, 3])
