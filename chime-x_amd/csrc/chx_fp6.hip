@@ -31,7 +31,9 @@
 //     column operands straight into VGPRs, three stages ahead, so every
 //     operand byte crosses the CU's load path once.  The load path bounds
 //     the kernel: each load instruction holds up the wave's in-order issue,
-//     and a DMA costs about 1.5x a direct load of the same bytes.
+//     and a DMA costs about 1.5x a direct load of the same bytes.  The half
+//     tiles at the array's right edge go two to a workgroup (one 128-row
+//     block per pair of waves, each with its own ring).
 #include "chx_internal.h"
 
 namespace chx {
