@@ -546,7 +546,7 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
 __global__ __launch_bounds__(256, 1) void corr_f6_kernel(GemmJob J, int head, int tail,
                                                          int tmode) {
   __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * kRing][kSlotBytes];  // two rings
-  const int64_t per_f = fp6_tiles(J.ntiles);
+  const int64_t per_f = paired_tiles(J.ntiles);
   int64_t item;
   int s0, ns;
   if ((int)blockIdx.x < head) {
@@ -657,7 +657,7 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
   const int N = p->cfg.num_elements, F = p->cfg.num_freq, U = p->cfg.upper_triangle_convention;
   const int64_t T = p->cfg.time_steps;
   const int ntiles = p->ntiles, Np = ntiles * kTile;
-  const int64_t items = (int64_t)F * fp6_tiles(ntiles);
+  const int64_t items = (int64_t)F * paired_tiles(ntiles);
   if (items > 0x7FFFFFFF || !p->d_work) return CHX_ERR_SHAPE;
   int cus = 0;
   CHX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->cfg.device));
@@ -697,7 +697,7 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
     const int tmode = split > 1 ? 2 : kmode;  // the tail's pieces add atomically
     if (tail > 0 && split > 1 && kmode == 0) {
       // zero the channels the tail touches (before the head's stores)
-      const int64_t f0 = head / fp6_tiles(ntiles);
+      const int64_t f0 = head / paired_tiles(ntiles);
       int rc = launch_fill_u32((uint32_t *)vis + f0 * p->NB * 2048, 0u,
                                (int64_t)(F - f0) * p->NB * 2048, s);
       if (rc) return rc;

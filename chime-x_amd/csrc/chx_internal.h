@@ -77,10 +77,10 @@ __host__ __device__ inline int64_t wide_tiles(int64_t ntiles) {
   const int64_t m = ntiles / 2;
   return ntiles % 2 ? (m + 1) * (m + 1) : m * (m + 1);
 }
-// FP6 work items per channel: the (ntiles + 1) / 2 half tiles (the last
-// tile of each row tile ti with ntiles - ti odd, whose right half lies past
-// the array) go two to an item.
-__host__ __device__ inline int64_t fp6_tiles(int64_t ntiles) {
+// Work items per channel: the (ntiles + 1) / 2 half tiles (the last tile
+// of each row tile ti with ntiles - ti odd, whose right half lies past the
+// array) go two to an item.
+__host__ __device__ inline int64_t paired_tiles(int64_t ntiles) {
   return wide_tiles(ntiles) - (ntiles + 1) / 2 / 2;
 }
 size_t fp6_workspace_bytes(const chx_plan *p);
@@ -95,10 +95,9 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
 //        64-sample blocks, 1.5 bytes per (element, sample).
 enum PlaneFormat { kPlanesInt8 = 0, kPlanesFp6 = 1 };
 inline int format_ring(int fmt) { return fmt == kPlanesFp6 ? kFp6Ring : kMfmaRing; }
-// work items of one channel (tiles of 128 x 256; FP6 pairs half tiles)
-inline int64_t format_tiles(int fmt, int64_t ntiles) {
-  return fmt == kPlanesFp6 ? fp6_tiles(ntiles) : wide_tiles(ntiles);
-}
+// work items of one channel (both formats: tiles of 128 x 256, half tiles
+// in pairs)
+inline int64_t format_tiles(int, int64_t ntiles) { return paired_tiles(ntiles); }
 // the most stages one work item may cover
 inline int format_max_stages(int) { return kMaxItemStages; }
 // stages per work item of an nst-stage chunk cut at max_st (0 = whole)

@@ -48,6 +48,7 @@ constexpr int kRowGroups = 4;     // 32-element row groups per k-step (in LDS)
 // (no bank conflicts); one 64-lane buffer->LDS load fills exactly one
 // operand.
 constexpr int kSlotBytes = 2 * kRowGroups * 1024;  // 8 KB
+constexpr int kRingBytes = kRing * kSlotBytes;     // a ring of one row block
 
 // 4x4 byte transpose: r[s] holds byte e of sample s; c[e] gets byte s of
 // element e.  v_perm_b32 selects bytes of {s0:s1} (selector 0-3 = s1,
@@ -203,33 +204,48 @@ __global__ __launch_bounds__(256) void unpack_batch_kernel(const UnpackJob *__re
               J.kb_dst0 + kb, J.kb_dst0, f, stage);
 }
 
-// Tile tb of one channel: rows [kTile ti, +kTile), columns [kTile ct,
-// +2 kTile), row-major over ti, then ct = ti, ti + 2, ...  Row ti holds
-// ceil((ntiles - ti) / 2) tiles; the last one's second column half lies past
-// the array when ntiles - ti is odd.
-__device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti, int &ct) {
+// Tile tb of one channel.  First the full 128 x 256 tiles, row by row: row
+// tile ti holds floor((ntiles - ti) / 2) of them, columns from kTile ct, ct
+// = ti, ti + 2, ...  Then the half tiles, the last tile of each row tile ti
+// with ntiles - ti odd (ti = ntiles-1, ntiles-3, ...), whose 128 columns
+// are column tile ntiles-1: two of them form one item (rows ti_a for waves
+// 0-1, ti_b = ti_a - 2 for waves 2-3); an odd last one stays alone (ti_b <
+// 0; its waves 2-3 compute past the array).  Full tiles return ti_b = ti_a.
+__device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti_a, int &ti_b,
+                                            int &ct) {
   int r = 0;
-  while (tb >= (ntiles - r + 1) / 2) {
-    tb -= (ntiles - r + 1) / 2;
+  while (r < ntiles && tb >= (ntiles - r) / 2) {
+    tb -= (ntiles - r) / 2;
     r++;
   }
-  ti = r;
-  ct = r + 2 * tb;
+  if (r < ntiles) {
+    ti_a = ti_b = r;
+    ct = r + 2 * tb;
+  } else {
+    ct = ntiles - 1;
+    ti_a = ntiles - 1 - 4 * tb;
+    ti_b = ti_a - 2;
+  }
 }
 
-// Pass 2, one output tile (rows from kTile ti, columns from kTile ct) of
-// channel f over 64-sample stages [st_begin, st_begin + nst) (nst a multiple
-// of kMfmaRing), run as 32-sample k-steps.  Wave w computes all 128 rows x
-// columns 64 w.. as 4x2 blocks of two int32 accumulators (256 AGPRs):
+// Pass 2, one output item of channel f over 64-sample stages [st_begin,
+// st_begin + nst) (nst a multiple of kMfmaRing), run as 32-sample k-steps.
+// Wave w computes 128 rows x 64 columns as 4x2 blocks of two int32
+// accumulators (256 AGPRs):
+//   !PAIR: rows from kTile ti_a, columns from kTile ct + 64 w (a 128 x 256
+//          tile; the four waves share the rows);
+//    PAIR: two half tiles of column tile ct: waves 0-1 rows from kTile
+//          ti_a, waves 2-3 rows from kTile ti_b, columns kTile ct + 64 (w%2);
 // RE += Ar.Br + Ai.Bi,  IM += Ar.Bi + Ai.(-Br),  where -Br is negated per
 // byte in registers (exact: every byte is 8 x value, in [-64, 56]).  The
 // row operands, shared by the four waves, come through the LDS ring by
 // buffer->LDS DMA; each wave loads its own column operands straight into
 // VGPRs, three k-steps ahead, so every operand byte crosses the CU's load
 // path once.
-__device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const int ti,
-                                          const int ct, const int st_begin, const int nst,
-                                          uint8_t (*lds)[kSlotBytes]) {
+template <bool PAIR>
+__device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const int ti_a,
+                                          const int ti_b, const int ct, const int st_begin,
+                                          const int nst, uint8_t (*lds)[kSlotBytes]) {
   const int F = J.F, Np = J.Np, KB = J.KB, nb = J.nb, U = J.U, mode = J.mode;
   const int64_t NB = J.NB;
   int32_t *vis = J.vis;
@@ -246,19 +262,24 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
     r[pl] = __builtin_amdgcn_make_buffer_rsrc(
         (void *)(J.planes + ((int64_t)pl * F + f) * KB * blk_bytes), (short)0,
         (int)(KB * blk_bytes), 0x00020000);
-  // Row DMAs: wave w fills plane w/2, row groups 2(w&1), 2(w&1)+1 of each
-  // k-step.  Row group g holds elements kTile ti + 32 g (always inside the
-  // array).
-  const int lp = wave >> 1, gh = wave & 1;
-  int voff[2];
+  // Row DMAs.  !PAIR: wave w fills plane w/2, row groups 2(w&1), 2(w&1)+1
+  // of the one row block.  PAIR: wave w fills plane w&1, all four row
+  // groups of row block w/2 (rows ti_a, ti_b; ring w/2 of two).  Row group
+  // g holds elements kTile ti + 32 g (always inside the array).
+  constexpr int kDma = PAIR ? 4 : 2;  // row DMAs per wave and k-step
+  const int rb = PAIR ? wave >> 1 : 0;          // this wave's row block
+  const int lp = PAIR ? wave & 1 : wave >> 1;   // plane it fills
+  const int g0 = PAIR ? 0 : 2 * (wave & 1);     // first row group it fills
+  const int ti = PAIR && rb ? ti_b : ti_a;      // this wave's row tile
+  int voff[kDma];
 #pragma unroll
-  for (int gg = 0; gg < 2; gg++)
-    voff[gg] = (ti * kTile + 32 * (2 * gh + gg) + (lane & 31)) * 16 + (lane >> 5) * (int)blk_bytes;
+  for (int gg = 0; gg < kDma; gg++)
+    voff[gg] = (ti * kTile + 32 * (g0 + gg) + (lane & 31)) * 16 + (lane >> 5) * (int)blk_bytes;
   auto issue_rows = [&](int ks, int slot) {
     const int soff = (k0 + ks) * 2 * (int)blk_bytes;
 #pragma unroll
-    for (int gg = 0; gg < 2; gg++) {
-      lds_void *dst = (lds_void *)&lds[slot][(lp * kRowGroups + 2 * gh + gg) * 1024];
+    for (int gg = 0; gg < kDma; gg++) {
+      lds_void *dst = (lds_void *)&lds[rb * kRing + slot][(lp * kRowGroups + g0 + gg) * 1024];
       __builtin_amdgcn_raw_ptr_buffer_load_lds(r[lp], dst, 16, voff[gg], soff, 0, 0);
     }
   };
@@ -266,14 +287,15 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
   // (the right half of a row's last tile when the tile count is odd: waves
   // 2 and 3) load through an empty descriptor, which fetches nothing and
   // returns zeros; their products are never stored.
-  const bool past = wave >= 2 && (ct + 1) * kTile >= Np;
+  const bool past = !PAIR && wave >= 2 && (ct + 1) * kTile >= Np;
+  const int cw = PAIR ? wave & 1 : wave;  // column quarter of the tile
   const __amdgpu_buffer_rsrc_t rz = __builtin_amdgcn_make_buffer_rsrc(
       (void *)J.planes, (short)0, 0, 0x00020000);
   const __amdgpu_buffer_rsrc_t c0 = past ? rz : r[0], c1 = past ? rz : r[1];
   int boff[2];
 #pragma unroll
   for (int j = 0; j < 2; j++) {
-    int e = ct * kTile + 64 * wave + 32 * j;
+    int e = ct * kTile + 64 * cw + 32 * j;
     if (e >= Np) e -= kTile;
     boff[j] = (e + (lane & 31)) * 16 + (lane >> 5) * (int)blk_bytes;
   }
@@ -292,7 +314,7 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
 #pragma unroll
     for (int j = 0; j < 2; j++) re[i][j] = im[i][j] = zero;
   const uint8_t *lds0 = &lds[0][0];
-  const int bA = lane * 16;
+  const int bA = rb * kRingBytes + lane * 16;
   auto load_op = [&](int slot, int plane, int k) -> v4i {
     return *(const v4i *)(lds0 + bA + (slot * kSlotBytes + plane * kRowGroups * 1024 + k * 1024));
   };
@@ -333,13 +355,13 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
     ar3n = ar[3];
     ai3n = ai[3];
   };
-  const int by0 = ti * kTile / 32, bx0 = (ct * kTile + 64 * wave) / 32;
+  const int by0 = ti * kTile / 32, bx0 = (ct * kTile + 64 * cw) / 32;
 
   // Issue groups: G(ks), issued during k-step ks, = the row DMAs of k-step
-  // ks+4 (2) and the column loads of ks+3 (4).  Prologue: G(-4) = DMAs of
-  // k-step 0, G(-3..-1) = DMAs of 1..3 and column loads of 0..2 (sched
+  // ks+4 (kDma) and the column loads of ks+3 (4).  Prologue: G(-4) = DMAs
+  // of k-step 0, G(-3..-1) = DMAs of 1..3 and column loads of 0..2 (sched
   // barriers keep the issue order the waits count on).  Wait for this
-  // wave's row DMAs of k-step 0 (18 later operations may stay pending),
+  // wave's row DMAs of k-step 0 (3 |G| later operations may stay pending),
   // barrier, read them.  Column operands of k-step s live in set s % 4:
   // k-step ks loads ks+3 into the set k-step ks-1 has finished.
   static_assert(kRing == 4, "column loads three k-steps ahead, row DMAs four");
@@ -356,7 +378,10 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
   issue_rows(cl(3), 3);
   load_cols(BR[2][0], BR[2][1], BI[2][0], BI[2][1], cl(2));
   __builtin_amdgcn_sched_barrier(0);
-  asm volatile("s_waitcnt vmcnt(18)\n\ts_barrier" ::: "memory");
+  if constexpr (PAIR)
+    asm volatile("s_waitcnt vmcnt(24)\n\ts_barrier" ::: "memory");
+  else
+    asm volatile("s_waitcnt vmcnt(18)\n\ts_barrier" ::: "memory");
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     ar[i] = load_op(0, 0, i);
@@ -367,7 +392,7 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
 
   // K-step ks (slot ks % kRing):
   //   wait until G(ks-3) landed (this wave's DMAs of ks+1 and column loads
-  //   of ks; G(ks-2), G(ks-1) may stay pending), barrier: every wave's DMAs
+  //   of ks; G(ks-2), G(ks-1), 2 |G| operations, may stay pending), barrier: every wave's DMAs
   //   of ks+1 landed and every wave's reads of slot ks are done; refill
   //   slot ks with k-step ks+4, load the columns of ks+3; MFMAs of ks,
   //   reading ks+1's row operands as ks's die.
@@ -376,23 +401,26 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
     for (int d = 0; d < kRing; d++) {
       const int ks = k00 + d, n3 = (d + 3) % kRing;
       __builtin_amdgcn_sched_barrier(0);
-      asm volatile("s_waitcnt vmcnt(12)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+      if constexpr (PAIR)
+        asm volatile("s_waitcnt vmcnt(16)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+      else
+        asm volatile("s_waitcnt vmcnt(12)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
       __builtin_amdgcn_sched_barrier(0);
       issue_rows(cl(ks + 4), d);
       load_cols(BR[n3][0], BR[n3][1], BI[n3][0], BI[n3][1], cl(ks + 3));
       step_ops(BR[d][0], BR[d][1], BI[d][0], BI[d][1], (d + 1) % kRing);
-      // 8 MFMAs per row group i; VMEM ops (2 DMAs, 4 column loads) over
+      // 8 MFMAs per row group i; VMEM ops (kDma DMAs, 4 column loads) over
       // groups 0-1, LDS reads of the next row operands from group 1 on
 #pragma unroll
       for (int i = 0; i < 8; i++) {
         __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-        if (i < 3) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+        if (i < (PAIR ? 4 : 3)) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
       }
 #pragma unroll
       for (int i = 0; i < 8; i++) {
         __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
         if (i < 2) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-        if (i >= 4 && i < 7) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+        if (i >= 4 && i < (PAIR ? 8 : 7)) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
       }
 #pragma unroll
       for (int i = 0; i < 8; i++) {
@@ -428,8 +456,8 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
 // piece the rest), mode tmode.  Each range is XCD-remapped.
 __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(GemmJob J, int nst, int head,
                                                            int tail, int tmode) {
-  __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing][kSlotBytes];
-  const int64_t per_f = wide_tiles(J.ntiles);
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * kRing][kSlotBytes];  // two rings
+  const int64_t per_f = paired_tiles(J.ntiles);
   int64_t item;
   int s0, ns;
   if ((int)blockIdx.x < head) {
@@ -444,9 +472,12 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(GemmJob J, int nst, i
     ns = nst - s0 < J.stages_per_split ? nst - s0 : J.stages_per_split;
     J.mode = tmode;
   }
-  int ti, ct;
-  tile_coords((int)(item % per_f), J.ntiles, ti, ct);
-  gemm_tile(J, (int)(item / per_f), ti, ct, s0, ns, lds);
+  int ti_a, ti_b, ct;
+  tile_coords((int)(item % per_f), J.ntiles, ti_a, ti_b, ct);
+  if (ti_b >= 0 && ti_b != ti_a)
+    gemm_tile<true>(J, (int)(item / per_f), ti_a, ti_b, ct, s0, ns, lds);
+  else
+    gemm_tile<false>(J, (int)(item / per_f), ti_a, ti_a, ct, s0, ns, lds);
 }
 
 // Batch: persistent workgroups take work items from 8 per-XCD queues: a
@@ -458,8 +489,8 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_batch_kernel(
   // One LDS object: the ring and, past it, the item handed to the
   // workgroup.  (A second __shared__ variable makes the compiler wait for
   // every buffer->LDS load before each LDS read in the loop.)
-  __shared__ __attribute__((aligned(1024))) uint8_t lds[kRing + 1][kSlotBytes];
-  int &next = *(int *)&lds[kRing][0];
+  __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * kRing + 1][kSlotBytes];
+  int &next = *(int *)&lds[2 * kRing][0];
   int q = blockIdx.x % 8, moves = 0;
   for (;;) {
     if (threadIdx.x == 0) {
@@ -481,9 +512,12 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_batch_kernel(
     if (it < 0) break;
     const GemmItem w = items[it];
     const GemmJob J = jobs[w.job];
-    int ti, ct;
-    tile_coords(w.tile, J.ntiles, ti, ct);
-    gemm_tile(J, w.f, ti, ct, w.stage0, w.nstages, lds);
+    int ti_a, ti_b, ct;
+    tile_coords(w.tile, J.ntiles, ti_a, ti_b, ct);
+    if (ti_b >= 0 && ti_b != ti_a)
+      gemm_tile<true>(J, w.f, ti_a, ti_b, ct, w.stage0, w.nstages, lds);
+    else
+      gemm_tile<false>(J, w.f, ti_a, ti_a, ct, w.stage0, w.nstages, lds);
   }
 }
 
@@ -516,7 +550,7 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
   const int N = p->cfg.num_elements, F = p->cfg.num_freq, U = p->cfg.upper_triangle_convention;
   const int64_t T = p->cfg.time_steps;
   const int ntiles = p->ntiles, Np = ntiles * kTile;
-  const int64_t per_f = wide_tiles(ntiles);
+  const int64_t per_f = paired_tiles(ntiles);
   const int64_t items = (int64_t)F * per_f;
   if (items > 0x7FFFFFFF || !p->d_work) return CHX_ERR_SHAPE;
   int cus = 0;

@@ -161,15 +161,15 @@ def _fake(jobs):
 
 
 def _tiles(nt, method):
-    """Work items of one channel: tiles of 128 x 256, row tile i holding
-    ceil((nt - i) / 2) of them; FP6 (method 3) puts the half tiles (the last
-    one of each row tile with nt - i odd) two to an item."""
+    """Work items of one channel (both matrix-core methods): tiles of
+    128 x 256, row tile i holding ceil((nt - i) / 2) of them, with the half
+    tiles (the last one of each row tile with nt - i odd) two to an item."""
     wide = sum((nt - i + 1) // 2 for i in range(nt))
-    return wide - (nt + 1) // 2 // 2 if method == 3 else wide
+    return wide - (nt + 1) // 2 // 2
 
 
-def _fp6_item_blocks(nt):
-    """The 32x32 blocks each FP6 work item of one channel computes, following
+def _item_blocks(nt):
+    """The 32x32 blocks each work item of one channel stores, following
     the kernel's item order: full tiles row by row, then pairs of half tiles
     (rows nt-1-4q and nt-3-4q, column tile nt-1), an odd last one alone."""
     items = []
@@ -206,17 +206,16 @@ def test_tile_count_closed_forms():
     for nt in range(1, 40):
         m = nt // 2
         wide = (m + 1) ** 2 if nt % 2 else m * (m + 1)
-        assert _tiles(nt, 2) == wide
-        assert _tiles(nt, 3) == wide - (nt + 1) // 2 // 2
+        assert _tiles(nt, 2) == _tiles(nt, 3) == wide - (nt + 1) // 2 // 2
 
 
-def test_fp6_items_cover_the_upper_triangle_once():
+def test_items_cover_the_upper_triangle_once():
     # Every upper-triangle 32x32 block of an nt-tile array is stored by
-    # exactly one FP6 work item (blocks below the diagonal are computed and
+    # exactly one work item (blocks below the diagonal are computed and
     # not stored; a lone half tile's columns past the array are not listed,
     # as they are never stored), and the item count is the planner's.
     for nt in range(1, 24):
-        items = _fp6_item_blocks(nt)
+        items = _item_blocks(nt)
         assert len(items) == _tiles(nt, 3)
         seen = {}
         for k, blocks in enumerate(items):
