@@ -322,9 +322,9 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   // Row DMAs.  !PAIR: wave w fills plane w/2, row groups 2(w&1), 2(w&1)+1
   // of the one row block: two P0 operands and one P1 group pair per stage.
   // PAIR: wave w fills plane w&1, all four row groups of row block w/2
-  // (rows ti_a, ti_b; ring w/2 of two): four P0 operands
-  // and two P1 group pairs.  Row group g holds elements kTile ti + 32 g
-  // (always inside the array).
+  // (rows ti_a, ti_b; ring w/2 of two): four P0 operands and two P1 group
+  // pairs.  Row group g holds elements kTile ti + 32 g (always inside the
+  // array).
   constexpr int kDma = PAIR ? 6 : 3;  // row DMAs per wave and stage
   constexpr int kP0n = PAIR ? 4 : 2, kP1n = PAIR ? 2 : 1;
   const int rb = PAIR ? wave >> 1 : 0;               // this wave's row block
@@ -358,11 +358,10 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
                                                0);
     }
   };
-  // Column operands of this wave: groups 2 w + j, lane -> (half lane/32,
-  // element + lane%32).  Columns past the array (the right half of a row's
-  // last tile when the tile count is odd: waves 2 and 3) load through an
-  // empty descriptor, which fetches nothing and returns zeros; their
-  // products are never stored.
+  // Column operands of this wave: column groups 2 cw + j of the tile, lane
+  // -> (half lane/32, element + lane%32).  Columns past the array (waves 2
+  // and 3 of an unpaired half tile) load through an empty descriptor, which
+  // fetches nothing and returns zeros; their products are never stored.
   const bool past = !PAIR && wave >= 2 && (ct + 1) * kTile >= Np;
   const int cw = PAIR ? wave & 1 : wave;  // column quarter of the tile
   const __amdgpu_buffer_rsrc_t rz = __builtin_amdgcn_make_buffer_rsrc(

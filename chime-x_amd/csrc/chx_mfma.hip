@@ -283,10 +283,10 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
       __builtin_amdgcn_raw_ptr_buffer_load_lds(r[lp], dst, 16, voff[gg], soff, 0, 0);
     }
   };
-  // Column operands of this wave: groups 2 w + j.  Columns past the array
-  // (the right half of a row's last tile when the tile count is odd: waves
-  // 2 and 3) load through an empty descriptor, which fetches nothing and
-  // returns zeros; their products are never stored.
+  // Column operands of this wave: column groups 2 cw + j of the tile.
+  // Columns past the array (waves 2 and 3 of an unpaired half tile) load
+  // through an empty descriptor, which fetches nothing and returns zeros;
+  // their products are never stored.
   const bool past = !PAIR && wave >= 2 && (ct + 1) * kTile >= Np;
   const int cw = PAIR ? wave & 1 : wave;  // column quarter of the tile
   const __amdgpu_buffer_rsrc_t rz = __builtin_amdgcn_make_buffer_rsrc(
