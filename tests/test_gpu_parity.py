@@ -185,6 +185,19 @@ def test_fp6_uneven_time_pieces():
     np.testing.assert_array_equal(got, O.correlate_blocks(data, 1))
 
 
+@pytest.mark.parametrize("method", MATRIX)
+def test_half_tile_pairs_against_oracle(method):
+    # Arrays whose row tiles end in half tiles: N=512 (4 tiles: one pair),
+    # 640 (5 tiles: a pair and a lone half tile), 1152 (9 tiles: two pairs
+    # and a lone one), 1376 (11 tiles, N not a multiple of 128: three
+    # pairs), checked against the oracle for both conventions.
+    for N, seed in ((512, 51), (640, 64), (1152, 11), (1376, 13)):
+        data = rand(256, 2, N, seed)
+        for U in (1, 0):
+            np.testing.assert_array_equal(run(data, method, U, 32), O.correlate_blocks(data, U),
+                                          err_msg=f"N={N} U={U}")
+
+
 def test_methods_agree_on_array_scale_channel_slice():
     # N=2048 (config 2's array) on a T slice: all three methods bit-exact
     # against each other and against the oracle.
