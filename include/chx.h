@@ -172,7 +172,7 @@ typedef struct chx_batch_tuning {
   int32_t max_item_stages;  /* time extent of one work item, in 64-sample
                                stages (rounded up to a multiple of the
                                method's ring, 4 stages); 0 = a whole chunk
-                               per item (FP6: at most 2048 stages, longer
+                               per item (at most 2048 stages; longer
                                chunks are cut) */
   int32_t workgroups;       /* persistent workgroups (multiple of 8);
                                0 = one per compute unit */
