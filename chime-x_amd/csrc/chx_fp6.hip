@@ -327,6 +327,7 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   // array).
   constexpr int kDma = PAIR ? 6 : 3;  // row DMAs per wave and stage
   constexpr int kP0n = PAIR ? 4 : 2, kP1n = PAIR ? 2 : 1;
+  static_assert(kDma == kP0n + kP1n, "the waits below count kDma + 8 loads per stage");
   const int rb = PAIR ? wave >> 1 : 0;               // this wave's row block
   const int lp = PAIR ? wave & 1 : wave >> 1;        // plane it fills
   const int g0 = PAIR ? 0 : 2 * (wave & 1);          // first row group it fills
