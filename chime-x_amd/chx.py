@@ -163,6 +163,8 @@ def lib() -> ctypes.CDLL:
     L.chx_blocks_to_triangle_device.argtypes = [i32, i32, vp, vp, vp]
     L.chx_blocks_to_full.argtypes = [i32, i32, vp, vp]
     L.chx_blocks_to_full_device.argtypes = [i32, i32, vp, vp, vp]
+    L.chx_split_paired_feeds.argtypes = [i32, i32, vp]
+    L.chx_feed16_to_triangle.argtypes = [i32, i32, vp, vp]
     L.chx_strerror.argtypes = [ctypes.c_int]
     L.chx_strerror.restype = ctypes.c_char_p
     L.chx_last_error_detail.restype = ctypes.c_char_p
@@ -244,6 +246,29 @@ def blocks_to_full_device(num_freq: int, N: int, d_blocks: int, d_full: int,
                           stream: int = 0) -> None:
     _check(lib().chx_blocks_to_full_device(num_freq, N, d_blocks, d_full, stream or None),
            "chx_blocks_to_full_device")
+
+
+def split_paired_feeds(full: np.ndarray, num_freq: int, E: int = 16) -> np.ndarray:
+    """16-feed unpacking (chx_split_paired_feeds): [F//2][2E][2E][2] full
+    matrices of a paired run -> [2*(F//2)][E][E][2]."""
+    work = np.array(full, np.int32, order="C", copy=True)
+    fin = num_freq // 2
+    if work.size < fin * (2 * E) ** 2 * 2:
+        raise ValueError("full holds fewer than num_freq//2 matrices of 2E x 2E")
+    _check(lib().chx_split_paired_feeds(num_freq, E, _p(work)), "chx_split_paired_feeds")
+    return work.reshape(-1)[: fin * 2 * E * E * 2].reshape(2 * fin, E, E, 2).copy()
+
+
+def feed16_to_triangle(full16: np.ndarray, num_freq_out: int | None = None) -> np.ndarray:
+    """[F][16][16][2] -> [num_freq_out][136][2] (chx_feed16_to_triangle),
+    zero-padded past F."""
+    full16 = np.ascontiguousarray(full16, np.int32)
+    F = full16.size // 512
+    F_out = F if num_freq_out is None else num_freq_out
+    out = np.empty((F_out, 136, 2), np.int32)
+    _check(lib().chx_feed16_to_triangle(F_out, F, _p(full16), _p(out)),
+           "chx_feed16_to_triangle")
+    return out
 
 
 def device_count() -> int:

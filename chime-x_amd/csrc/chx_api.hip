@@ -158,6 +158,38 @@ int chx_blocks_to_full(int32_t num_freq, int32_t num_elements, const int32_t *bl
   return CHX_SUCCESS;
 }
 
+int chx_split_paired_feeds(int32_t num_freq, int32_t num_elements, int32_t *data) {
+  if (!data || num_freq < 0 || num_elements < 0) return CHX_ERR_INVALID_ARGUMENT;
+  // Per packed frequency g and row y of its 2E x 2E matrix, the E entries of
+  // the row's own half move down to the next free output row. The write
+  // cursor never passes the read cursor, so memmove keeps it in place.
+  const int64_t E = num_elements, row = 2 * E;  // complex entries per input row
+  int64_t w = 0;
+  for (int64_t g = 0; g < num_freq / 2; g++)
+    for (int64_t y = 0; y < row; y++) {
+      const int64_t r = ((g * row + y) * row + (y < E ? 0 : E)) * 2;
+      memmove(data + w, data + r, (size_t)E * 2 * sizeof(int32_t));
+      w += E * 2;
+    }
+  return CHX_SUCCESS;
+}
+
+int chx_feed16_to_triangle(int32_t num_freq_out, int32_t num_freq, const int32_t *input,
+                           int32_t *output) {
+  if (!input || !output || num_freq_out < 0 || num_freq < 0) return CHX_ERR_INVALID_ARGUMENT;
+  int32_t *o = output;
+  for (int64_t f = 0; f < num_freq_out; f++)
+    for (int y = 0; y < 16; y++) {
+      const size_t n = (size_t)(16 - y) * 2;
+      if (f < num_freq)
+        memcpy(o, input + (f * 256 + y * 17) * 2, n * sizeof(int32_t));
+      else
+        memset(o, 0, n * sizeof(int32_t));
+      o += n;
+    }
+  return CHX_SUCCESS;
+}
+
 int chx_blocks_to_full_device(int32_t num_freq, int32_t num_elements, const int32_t *d_blocks,
                               int32_t *d_full, void *stream) {
   if (!d_blocks || !d_full || num_freq < 1 || chx_num_blocks(num_elements) < 0)

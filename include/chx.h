@@ -350,6 +350,23 @@ int chx_blocks_to_full(int32_t num_freq, int32_t num_elements,
 int chx_blocks_to_full_device(int32_t num_freq, int32_t num_elements,
                               const int32_t *d_blocks, int32_t *d_full, void *stream);
 
+/* reorganize_32_to_16_feed_GPU_Correlated_Data (gpu_data_reorg.c:6-34), the
+ * 16-feed (pathfinder) unpacking. Bytes [T][F][E] with E = 16 read as
+ * [T][F/2][2E] put frequencies 2g and 2g+1 side by side in one 2E-element
+ * frequency; after a run at num_elements 2E, F/2 frequencies, and
+ * chx_blocks_to_full, this keeps the two diagonal quadrants of each 2E x 2E
+ * matrix: in place, [F/2][2E][2E][2] -> [2(F/2)][E][E][2] (frequency 2g = the
+ * rows and columns < E). F/2 rounds down; entries past the output keep their
+ * input values. Any E >= 0 (the reference allows any). */
+int chx_split_paired_feeds(int32_t num_freq, int32_t num_elements, int32_t *data);
+
+/* reorganize_data_16_element_with_triangle_conversion (gpu_data_reorg.c:
+ * 126-149): [num_freq][16][16][2] full matrices -> [num_freq_out][136][2]
+ * upper triangles (row y, column x >= y, row-major); frequencies num_freq ..
+ * num_freq_out-1 are zero. input and output must not overlap. */
+int chx_feed16_to_triangle(int32_t num_freq_out, int32_t num_freq, const int32_t *input,
+                           int32_t *output);
+
 /* Device version of the same reorg (no reference counterpart: it keeps the
  * visibilities on the GPU for consumers that want the triangle). */
 int chx_blocks_to_triangle_device(int32_t num_freq, int32_t num_elements,

@@ -273,6 +273,37 @@ void chxo_blocks_to_full(int F, int N, const int32_t *blocks, int32_t *full) {
   }
 }
 
+/* reorganize_32_to_16_feed_GPU_Correlated_Data (gpu_data_reorg.c:6-34): in
+ * place over full matrices [F/2][2E][2E][2]; walks every input entry in
+ * order (:17-19) and keeps it when both indices lie in the same half
+ * (:20-27), so frequency 2g gets the (<E, <E) quadrant and 2g+1 the
+ * (>=E, >=E) one.  F/2 rounds down (:12). */
+void chxo_split_paired_feeds(int F, int E, int32_t *data) {
+  const int fin = F / 2, ein = 2 * E;
+  int64_t a = 0, o = 0;
+  for (int g = 0; g < fin; g++)
+    for (int y = 0; y < ein; y++)
+      for (int x = 0; x < ein; x++, a += 2) {
+        if ((x < E) != (y < E)) continue; /* :29-30 */
+        data[o++] = data[a];
+        data[o++] = data[a + 1];
+      }
+}
+
+/* reorganize_data_16_element_with_triangle_conversion (gpu_data_reorg.c:
+ * 126-149): [F_in][16][16][2] full matrices -> [F_out][136][2] upper
+ * triangles (row y, x >= y), zero for frequencies F_in .. F_out-1 (:141-145). */
+void chxo_feed16_to_triangle(int F_out, int F_in, const int32_t *in, int32_t *out) {
+  int64_t o = 0;
+  for (int f = 0; f < F_out; f++)
+    for (int y = 0; y < 16; y++)
+      for (int x = y; x < 16; x++) {
+        const int64_t i = ((int64_t)f * 256 + y * 16 + x) * 2; /* :134 */
+        out[o++] = f < F_in ? in[i] : 0;
+        out[o++] = f < F_in ? in[i + 1] : 0;
+      }
+}
+
 /* offset_accumulator.cl:7-49 */
 void chxo_offset_sums(int64_t T, int F, int N, const uint8_t *in, uint32_t *sums) {
   const int64_t row = (int64_t)F * N;

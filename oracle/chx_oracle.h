@@ -71,6 +71,13 @@ void chxo_block_maps(int N, uint32_t *x_map, uint32_t *y_map);
 void chxo_blocks_to_triangle(int F, int N, const int32_t *blocks, int32_t *tri);
 void chxo_blocks_to_full(int F, int N, const int32_t *blocks, int32_t *full);
 
+/* gpu_data_reorg.c:6-34: two 16-feed frequencies packed as one 2E-element
+ * frequency -> [F][E][E][2], in place (F/2 input frequencies). */
+void chxo_split_paired_feeds(int F, int E, int32_t *data);
+/* gpu_data_reorg.c:126-149: 16x16 full matrices -> 136-entry triangles,
+ * zero-padded to F_out frequencies. */
+void chxo_feed16_to_triangle(int F_out, int F_in, const int32_t *in, int32_t *out);
+
 /* offset_accumulator.cl:7-49: sums[F][N][2] = (sum_t R, sum_t I) of the raw
  * nibbles, added into sums (the host zeroes it, main_wrapper.c:671-679). */
 void chxo_offset_sums(int64_t T, int F, int N, const uint8_t *in, uint32_t *sums);

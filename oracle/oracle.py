@@ -47,6 +47,8 @@ def lib() -> ctypes.CDLL:
         L.chxo_blocks_to_triangle.argtypes = [i32, i32, vp, vp]
         L.chxo_blocks_to_full.argtypes = [i32, i32, vp, vp]
         L.chxo_offset_sums.argtypes = [i64, i32, i32, vp, vp]
+        L.chxo_split_paired_feeds.argtypes = [i32, i32, vp]
+        L.chxo_feed16_to_triangle.argtypes = [i32, i32, vp, vp]
         L.chxo_preseed.argtypes = [i32, i32, i64, i32, i32, vp, vp]
         L.chxo_packed_raw.argtypes = [i32, i32, i64, i32, i32, i32, vp, vp]
         L.chxo_compare_triangle.argtypes = [i32, i32, vp, vp, vp, vp]
@@ -123,6 +125,23 @@ def blocks_to_full(blocks: np.ndarray, N: int) -> np.ndarray:
     full = np.zeros((F, N, N, 2), np.int32)
     lib().chxo_blocks_to_full(F, N, _p(blocks), _p(full))
     return full
+
+
+def split_paired_feeds(full: np.ndarray, F: int, E: int) -> np.ndarray:
+    """[F//2][2E][2E][2] full matrices -> [F//2*2][E][E][2] (a copy)."""
+    work = np.ascontiguousarray(full, np.int32).copy()
+    assert work.size >= (F // 2) * (2 * E) ** 2 * 2
+    lib().chxo_split_paired_feeds(F, E, _p(work))
+    return work.reshape(-1)[: (F // 2) * 2 * E * E * 2].reshape(F // 2 * 2, E, E, 2).copy()
+
+
+def feed16_to_triangle(full16: np.ndarray, F_out: int) -> np.ndarray:
+    """[F_in][16][16][2] -> [F_out][136][2], zero past F_in."""
+    full16 = np.ascontiguousarray(full16, np.int32)
+    F_in = full16.size // 512
+    out = np.empty((F_out, 136, 2), np.int32)
+    lib().chxo_feed16_to_triangle(F_out, F_in, _p(full16), _p(out))
+    return out
 
 
 def offset_sums(data: np.ndarray) -> np.ndarray:

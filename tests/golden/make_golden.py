@@ -16,6 +16,8 @@ Reference entry points used (file:line):
   cpu_data_generate_and_correlate_nonstandard_convention cpu_corr_test.c:11
   reorganize_GPU_to_upper_triangle                     gpu_data_reorg.c:76
   reorganize_GPU_to_full_Matrix_for_comparison         gpu_data_reorg.c:36
+  reorganize_32_to_16_feed_GPU_Correlated_Data         gpu_data_reorg.c:6
+  reorganize_data_16_element_with_triangle_conversion  gpu_data_reorg.c:126
   compare_NSquared_correlator_results_data_has_upper_triangle_only
                                                         cpu_corr_test.c:269
 """
@@ -48,6 +50,8 @@ def load_ref():
     L.reorganize_GPU_to_full_Matrix_for_comparison.argtypes = [i, i, i, i, vp, vp]
     L.compare_NSquared_correlator_results_data_has_upper_triangle_only.argtypes = [
         vp, vp, i, i, vp, vp, vp, vp, i]
+    L.reorganize_32_to_16_feed_GPU_Correlated_Data.argtypes = [i, i, vp]
+    L.reorganize_data_16_element_with_triangle_conversion.argtypes = [i, i, vp, vp]
     return L
 
 
@@ -111,6 +115,21 @@ def main() -> int:
         full = np.zeros((F, N, N, 2), np.int32)
         L.reorganize_GPU_to_full_Matrix_for_comparison(32, nb, F, N, p(blocks), p(full))
         out[f"reorgfull_N{N}_F{F}__full"] = full
+
+    # 16-feed helpers: signed values that name their own position.
+    for F, E in ((4, 16), (5, 16), (6, 7), (1, 16)):
+        fin = F // 2
+        full = (np.arange(max(fin, 1) * (2 * E) ** 2 * 2, dtype=np.int64) * 13
+                - 50000).astype(np.int32)
+        out[f"split_F{F}_E{E}__in"] = full.copy()
+        L.reorganize_32_to_16_feed_GPU_Correlated_Data(F, E, p(full))
+        out[f"split_F{F}_E{E}__out"] = full
+    for F_out, F_in in ((4, 4), (6, 3), (2, 5), (3, 0)):
+        src = (np.arange(max(F_in, 1) * 512, dtype=np.int64) * -7 + 3000).astype(np.int32)
+        tri = np.full(F_out * 272, 0x5A5A5A5A, np.int32)
+        L.reorganize_data_16_element_with_triangle_conversion(F_out, F_in, p(src), p(tri))
+        out[f"tri16_out{F_out}_in{F_in}__in"] = src
+        out[f"tri16_out{F_out}_in{F_in}__out"] = tri
 
     # Comparator: planted differences.
     N, F = 64, 2

@@ -326,3 +326,38 @@ def test_blocks_to_full_host_matches_reference(golden):
         np.testing.assert_array_equal(chx.blocks_to_full(blocks, N),
                                       golden[f"reorgfull_N{N}_F{F}__full"])
     assert chx.lib().chx_blocks_to_full(1, 48, None, None) == chx.ERR_INVALID_ARGUMENT
+
+
+def test_split_paired_feeds_matches_reference(golden):
+    for F, E in ((4, 16), (5, 16), (6, 7), (1, 16)):
+        buf = golden[f"split_F{F}_E{E}__in"].copy()
+        assert chx.lib().chx_split_paired_feeds(F, E, buf.ctypes.data) == 0
+        np.testing.assert_array_equal(buf, golden[f"split_F{F}_E{E}__out"], err_msg=f"F{F} E{E}")
+    full = np.random.default_rng(3).integers(-2**31, 2**31, size=(3, 24, 24, 2), dtype=np.int32)
+    np.testing.assert_array_equal(chx.split_paired_feeds(full, 7, 12),
+                                  O.split_paired_feeds(full, 7, 12))
+    L = chx.lib()
+    assert L.chx_split_paired_feeds(2, 16, None) == chx.ERR_INVALID_ARGUMENT
+    assert L.chx_split_paired_feeds(-2, 16, full.ctypes.data) == chx.ERR_INVALID_ARGUMENT
+    assert L.chx_split_paired_feeds(2, -1, full.ctypes.data) == chx.ERR_INVALID_ARGUMENT
+    before = full.copy()
+    assert L.chx_split_paired_feeds(1, 12, full.ctypes.data) == 0  # F/2 = 0: nothing
+    np.testing.assert_array_equal(full, before)
+
+
+def test_feed16_to_triangle_matches_reference(golden):
+    for F_out, F_in in ((4, 4), (6, 3), (2, 5), (3, 0)):
+        src = golden[f"tri16_out{F_out}_in{F_in}__in"]
+        out = np.full(F_out * 272, 0x5A5A5A5A, np.int32)
+        assert chx.lib().chx_feed16_to_triangle(F_out, F_in, src.ctypes.data,
+                                                out.ctypes.data) == 0
+        np.testing.assert_array_equal(out, golden[f"tri16_out{F_out}_in{F_in}__out"])
+    full16 = np.random.default_rng(4).integers(-2**31, 2**31, size=(5, 16, 16, 2), dtype=np.int32)
+    for F_out in (5, 2, 9):
+        np.testing.assert_array_equal(chx.feed16_to_triangle(full16, F_out),
+                                      O.feed16_to_triangle(full16, F_out))
+    L = chx.lib()
+    assert L.chx_feed16_to_triangle(1, 1, None, full16.ctypes.data) == chx.ERR_INVALID_ARGUMENT
+    assert L.chx_feed16_to_triangle(1, 1, full16.ctypes.data, None) == chx.ERR_INVALID_ARGUMENT
+    assert L.chx_feed16_to_triangle(-1, 1, full16.ctypes.data,
+                                    full16.ctypes.data) == chx.ERR_INVALID_ARGUMENT

@@ -219,6 +219,19 @@ def test_matrix_long_small_array_multi_chunk(method):
     np.testing.assert_array_equal(run(data, method, 1, 32), O.correlate_blocks(data, 1))
 
 
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("U", [1, 0])
+def test_paired_16_feed_run_against_oracle(method, U):
+    # A 16-feed array at F frequencies runs as 32 elements at F/2 (the bytes
+    # are read as they lie); the split and the 16-element triangles give
+    # each frequency's own 16-feed correlation.
+    T, F = 256, 8
+    data = rand(T, F, 16, 160 + U)
+    blocks = run(data.reshape(T, F // 2, 32), method, U, 64)
+    tri = chx.feed16_to_triangle(chx.split_paired_feeds(chx.blocks_to_full(blocks, 32), F, 16))
+    np.testing.assert_array_equal(tri, O.correlate_triangle(data, U))
+
+
 def test_full_matrix_expansion_on_device(golden):
     torch = _torch()
     for N, F in ((96, 2), (64, 1)):

@@ -215,3 +215,36 @@ def test_full_expansion_of_blocks_equals_reference_full_correlator(golden, golde
                                           golden[f"{case}__full_U{U}"], err_msg=f"{case} U={U}")
         done += 1
     assert done >= 5
+
+
+SPLIT_CASES = ((4, 16), (5, 16), (6, 7), (1, 16))   # (F, E)
+TRI16_CASES = ((4, 4), (6, 3), (2, 5), (3, 0))      # (F_out, F_in)
+
+
+def test_split_paired_feeds_matches_reference(golden):
+    # whole buffer, in place: the tail past the output keeps its input values
+    for F, E in SPLIT_CASES:
+        buf = golden[f"split_F{F}_E{E}__in"].copy()
+        O.lib().chxo_split_paired_feeds(F, E, O._p(buf))
+        np.testing.assert_array_equal(buf, golden[f"split_F{F}_E{E}__out"], err_msg=f"F{F} E{E}")
+
+
+def test_feed16_to_triangle_matches_reference(golden):
+    for F_out, F_in in TRI16_CASES:
+        src = golden[f"tri16_out{F_out}_in{F_in}__in"]
+        out = np.full(F_out * 272, 0x5A5A5A5A, np.int32)
+        O.lib().chxo_feed16_to_triangle(F_out, F_in, O._p(src), O._p(out))
+        np.testing.assert_array_equal(out, golden[f"tri16_out{F_out}_in{F_in}__out"])
+
+
+@pytest.mark.parametrize("U", [1, 0])
+def test_paired_feed_run_is_the_16_feed_correlation(U):
+    # Bytes [T][F][16] read as [T][F/2][32] hold frequencies 2g and 2g+1 side
+    # by side; the diagonal quadrants of the 32-element correlation are the
+    # 16-element correlations of those two frequencies.  Taking an
+    # off-diagonal quadrant, the wrong half, or the wrong triangle fails.
+    T, F = 40, 6
+    data = np.random.default_rng(16).integers(0, 256, size=(T, F, 16), dtype=np.uint8)
+    full = O.blocks_to_full(O.correlate_blocks(data.reshape(T, F // 2, 32), U), 32)
+    tri = O.feed16_to_triangle(O.split_paired_feeds(full, F, 16), F)
+    np.testing.assert_array_equal(tri, O.correlate_triangle(data, U))
