@@ -19,6 +19,16 @@ namespace chx {
 namespace {
 
 constexpr int kStage = 32;  // samples staged in LDS per step
+// LDS row lengths of the staged samples.  A 128-byte row padded by 16 bytes
+// starts 4 banks further on than the row before it, so the 16-byte stores
+// of the 8 lanes that write 4 consecutive rows at once (2 lanes per row)
+// hit 8 different groups of 4 banks; the same holds for the 256-byte column
+// rows padded by 16 bytes.  The column-pair rows keep the pairs (4tx, 4tx+1)
+// of the 8 column lanes in their first 128 bytes and (4tx+2, 4tx+3) in the
+// next 128, so one read of 8 lanes covers 128 consecutive bytes.
+constexpr int kRowU32 = 32 + 4;
+constexpr int kColPairs = 32 + 2;
+constexpr int kRxU32 = 16 + 4;
 
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
@@ -142,8 +152,9 @@ __global__ __launch_bounds__(64) void corr_conference_kernel(
     const uint8_t *__restrict__ in, int32_t *__restrict__ vis,
     const uint32_t *__restrict__ maps, int64_t NB, int N, int64_t row_bytes,
     int64_t t_slice, int ta, int atomic) {
-  __shared__ uint32_t py[kStage][32];  // (R<<16)|I of the row elements
-  __shared__ uint2 px[kStage][32];     // (R, I) of the column elements
+  // (R<<16)|I of the row elements; (R, I) of the column elements
+  __shared__ __attribute__((aligned(16))) uint32_t py[kStage][kRowU32];
+  __shared__ __attribute__((aligned(16))) uint2 px[kStage][kColPairs];
   const int lane = threadIdx.x;
   const int64_t b = blockIdx.x;
   const int f = blockIdx.y;
@@ -179,8 +190,8 @@ __global__ __launch_bounds__(64) void corr_conference_kernel(
           rx[k] = make_uint2(bx8 >> 4, bx8 & 15u);
         }
         *(uint4 *)&py[ls][16 * lh + 4 * q] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
-        *(uint4 *)&px[ls][16 * lh + 4 * q] = make_uint4(rx[0].x, rx[0].y, rx[1].x, rx[1].y);
-        *(uint4 *)&px[ls][16 * lh + 4 * q + 2] = make_uint4(rx[2].x, rx[2].y, rx[3].x, rx[3].y);
+        *(uint4 *)&px[ls][8 * lh + 2 * q] = make_uint4(rx[0].x, rx[0].y, rx[1].x, rx[1].y);
+        *(uint4 *)&px[ls][16 + 8 * lh + 2 * q] = make_uint4(rx[2].x, rx[2].y, rx[3].x, rx[3].y);
       }
     }
     __syncthreads();
@@ -188,8 +199,8 @@ __global__ __launch_bounds__(64) void corr_conference_kernel(
 #pragma unroll
       for (int j = 0; j < 8; j++) {
         const uint4 yv = *(const uint4 *)&py[s8 + j][4 * ty];
-        const uint4 x01 = *(const uint4 *)&px[s8 + j][4 * tx];
-        const uint4 x23 = *(const uint4 *)&px[s8 + j][4 * tx + 2];
+        const uint4 x01 = *(const uint4 *)&px[s8 + j][2 * tx];
+        const uint4 x23 = *(const uint4 *)&px[s8 + j][16 + 2 * tx];
         const uint32_t Py[4] = {yv.x, yv.y, yv.z, yv.w};
         const uint32_t Rx[4] = {x01.x, x01.z, x23.x, x23.z};
         const uint32_t Ix[4] = {x01.y, x01.w, x23.y, x23.w};
@@ -239,10 +250,11 @@ __global__ __launch_bounds__(64) void corr_highly_packed_kernel(
     const uint8_t *__restrict__ in, int32_t *__restrict__ vis,
     const uint32_t *__restrict__ maps, int64_t NB, int N, int64_t row_bytes,
     int64_t t_slice, int ta, int atomic) {
-  __shared__ uint32_t hy[kStage][32];   // P'(y)
-  __shared__ uint32_t hry[kStage][32];  // R_y
-  __shared__ uint32_t hx[kStage][32];   // P(x) = (R_x<<16) | I_x
-  __shared__ uint32_t hrx[kStage][16];  // (R_x(2m)<<16) | R_x(2m+1)
+  // P'(y); R_y; P(x) = (R_x<<16) | I_x; (R_x(2m)<<16) | R_x(2m+1)
+  __shared__ __attribute__((aligned(16))) uint32_t hy[kStage][kRowU32];
+  __shared__ __attribute__((aligned(16))) uint32_t hry[kStage][kRowU32];
+  __shared__ __attribute__((aligned(16))) uint32_t hx[kStage][kRowU32];
+  __shared__ __attribute__((aligned(16))) uint32_t hrx[kStage][kRxU32];
   const int lane = threadIdx.x;
   const int64_t b = blockIdx.x;
   const int f = blockIdx.y;
