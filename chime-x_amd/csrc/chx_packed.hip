@@ -18,12 +18,12 @@
 namespace chx {
 namespace {
 
-constexpr int kStage = 32;  // samples staged in LDS per step
+constexpr int kStage = 16;  // samples staged in LDS per step
 // LDS row lengths of the staged samples.  A 128-byte row padded by 16 bytes
 // starts 4 banks further on than the row before it, so the 16-byte stores
-// of the 8 lanes that write 4 consecutive rows at once (2 lanes per row)
-// hit 8 different groups of 4 banks; the same holds for the 256-byte column
-// rows padded by 16 bytes.  The column-pair rows keep the pairs (4tx, 4tx+1)
+// of the 8 lanes that write 2 consecutive rows at once (4 lanes per row, 32
+// bytes apart) hit 8 different groups of 4 banks; the same holds for the
+// 256-byte column rows padded by 16 bytes.  The column-pair rows keep the pairs (4tx, 4tx+1)
 // of the 8 column lanes in their first 128 bytes and (4tx+2, 4tx+3) in the
 // next 128, so one read of 8 lanes covers 128 consecutive bytes.
 constexpr int kRowU32 = 32 + 4;
@@ -160,10 +160,10 @@ __global__ __launch_bounds__(64) void corr_conference_kernel(
   const int f = blockIdx.y;
   const int bx = maps[b], by = maps[NB + b];
   const int ty = lane >> 3, tx = lane & 7;
-  const int ls = lane >> 1, lh = lane & 1;  // loader: sample, 16-byte half
+  const int ls = lane >> 2, lq = lane & 3;  // loader: sample, 8-byte quarter
   const int64_t t0 = (int64_t)blockIdx.z * t_slice;
-  const uint8_t *ysrc = in + (t0 + ls) * row_bytes + (int64_t)f * N + by * 32 + 16 * lh;
-  const uint8_t *xsrc = in + (t0 + ls) * row_bytes + (int64_t)f * N + bx * 32 + 16 * lh;
+  const uint8_t *ysrc = in + (t0 + ls) * row_bytes + (int64_t)f * N + by * 32 + 8 * lq;
+  const uint8_t *xsrc = in + (t0 + ls) * row_bytes + (int64_t)f * N + bx * 32 + 8 * lq;
 
   uint32_t a0[16], a1[16];
   int32_t re[16], im[16];
@@ -172,14 +172,14 @@ __global__ __launch_bounds__(64) void corr_conference_kernel(
   int cnt = 0;
 
   for (int64_t ts = 0; ts < t_slice; ts += kStage) {
-    const uint4 vy = *(const uint4 *)(ysrc + ts * row_bytes);
-    const uint4 vx = *(const uint4 *)(xsrc + ts * row_bytes);
+    const uint2 vy = *(const uint2 *)(ysrc + ts * row_bytes);
+    const uint2 vx = *(const uint2 *)(xsrc + ts * row_bytes);
     __syncthreads();
     {
-      const uint32_t wy[4] = {vy.x, vy.y, vy.z, vy.w};
-      const uint32_t wx[4] = {vx.x, vx.y, vx.z, vx.w};
+      const uint32_t wy[2] = {vy.x, vy.y};
+      const uint32_t wx[2] = {vx.x, vx.y};
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
+      for (int q = 0; q < 2; q++) {
         uint32_t pk[4];
         uint2 rx[4];
 #pragma unroll
@@ -189,9 +189,9 @@ __global__ __launch_bounds__(64) void corr_conference_kernel(
           pk[k] = ((by8 >> 4) << 16) | (by8 & 15u);
           rx[k] = make_uint2(bx8 >> 4, bx8 & 15u);
         }
-        *(uint4 *)&py[ls][16 * lh + 4 * q] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
-        *(uint4 *)&px[ls][8 * lh + 2 * q] = make_uint4(rx[0].x, rx[0].y, rx[1].x, rx[1].y);
-        *(uint4 *)&px[ls][16 + 8 * lh + 2 * q] = make_uint4(rx[2].x, rx[2].y, rx[3].x, rx[3].y);
+        *(uint4 *)&py[ls][8 * lq + 4 * q] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+        *(uint4 *)&px[ls][4 * lq + 2 * q] = make_uint4(rx[0].x, rx[0].y, rx[1].x, rx[1].y);
+        *(uint4 *)&px[ls][16 + 4 * lq + 2 * q] = make_uint4(rx[2].x, rx[2].y, rx[3].x, rx[3].y);
       }
     }
     __syncthreads();
@@ -260,10 +260,10 @@ __global__ __launch_bounds__(64) void corr_highly_packed_kernel(
   const int f = blockIdx.y;
   const int bx = maps[b], by = maps[NB + b];
   const int ty = lane >> 3, tx = lane & 7;
-  const int ls = lane >> 1, lh = lane & 1;
+  const int ls = lane >> 2, lq = lane & 3;
   const int64_t t0 = (int64_t)blockIdx.z * t_slice;
-  const uint8_t *ysrc = in + (t0 + ls) * row_bytes + (int64_t)f * N + by * 32 + 16 * lh;
-  const uint8_t *xsrc = in + (t0 + ls) * row_bytes + (int64_t)f * N + bx * 32 + 16 * lh;
+  const uint8_t *ysrc = in + (t0 + ls) * row_bytes + (int64_t)f * N + by * 32 + 8 * lq;
+  const uint8_t *xsrc = in + (t0 + ls) * row_bytes + (int64_t)f * N + bx * 32 + 8 * lq;
 
   uint32_t a0[8], a1[8], a2[8], ov[8];  // index yy*2 + m
   int32_t re[16], im[16];               // index yy*4 + xx
@@ -274,15 +274,14 @@ __global__ __launch_bounds__(64) void corr_highly_packed_kernel(
   int cnt = 0, since_harvest = 0;
 
   for (int64_t ts = 0; ts < t_slice; ts += kStage) {
-    const uint4 vy = *(const uint4 *)(ysrc + ts * row_bytes);
-    const uint4 vx = *(const uint4 *)(xsrc + ts * row_bytes);
+    const uint2 vy = *(const uint2 *)(ysrc + ts * row_bytes);
+    const uint2 vx = *(const uint2 *)(xsrc + ts * row_bytes);
     __syncthreads();
     {
-      const uint32_t wy[4] = {vy.x ^ 0x0F0F0F0Fu, vy.y ^ 0x0F0F0F0Fu, vy.z ^ 0x0F0F0F0Fu,
-                              vy.w ^ 0x0F0F0F0Fu};
-      const uint32_t wx[4] = {vx.x, vx.y, vx.z, vx.w};
+      const uint32_t wy[2] = {vy.x ^ 0x0F0F0F0Fu, vy.y ^ 0x0F0F0F0Fu};
+      const uint32_t wx[2] = {vx.x, vx.y};
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
+      for (int q = 0; q < 2; q++) {
         uint32_t pyv[4], ryv[4], pxv[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -292,10 +291,10 @@ __global__ __launch_bounds__(64) void corr_highly_packed_kernel(
           ryv[k] = by8 >> 4;
           pxv[k] = ((bx8 >> 4) << 16) | (bx8 & 15u);
         }
-        *(uint4 *)&hy[ls][16 * lh + 4 * q] = make_uint4(pyv[0], pyv[1], pyv[2], pyv[3]);
-        *(uint4 *)&hry[ls][16 * lh + 4 * q] = make_uint4(ryv[0], ryv[1], ryv[2], ryv[3]);
-        *(uint4 *)&hx[ls][16 * lh + 4 * q] = make_uint4(pxv[0], pxv[1], pxv[2], pxv[3]);
-        *(uint2 *)&hrx[ls][8 * lh + 2 * q] =
+        *(uint4 *)&hy[ls][8 * lq + 4 * q] = make_uint4(pyv[0], pyv[1], pyv[2], pyv[3]);
+        *(uint4 *)&hry[ls][8 * lq + 4 * q] = make_uint4(ryv[0], ryv[1], ryv[2], ryv[3]);
+        *(uint4 *)&hx[ls][8 * lq + 4 * q] = make_uint4(pxv[0], pxv[1], pxv[2], pxv[3]);
+        *(uint2 *)&hrx[ls][4 * lq + 2 * q] =
             make_uint2((pxv[0] & 0x000F0000u) | (pxv[1] >> 16),
                        (pxv[2] & 0x000F0000u) | (pxv[3] >> 16));
       }
@@ -440,7 +439,7 @@ int launch_corr_packed(const chx_plan *p, const uint8_t *in, int32_t *vis, hipSt
   const int N = p->cfg.num_elements, F = p->cfg.num_freq, ta = p->cfg.time_accum;
   const int64_t T = p->cfg.time_steps;
   const int64_t row_bytes = (int64_t)F * N;
-  // Slice length: a multiple of both the stage (32) and ta, so lane windows
+  // Slice length: a multiple of both the stage (16) and ta, so lane windows
   // start where the reference's chunks start.  Split T only as far as needed
   // for ~2048 wavefronts.
   int64_t unit = ta;
