@@ -8,7 +8,7 @@ output (16 x 2080 blocks, 260 MiB).  One step = one chx_correlate call (the
 whole integration of all 16 channels) through the C ABI.
 
 Prints ONE JSON line.  Both matrix-core methods are timed: int8 operands
-with int32 accumulation, and FP6 E3M2 operands (every 4-bit value is exact)
+with int32 accumulation, and FP6 E2M3 operands (every 4-bit value v exact as v/8)
 with FP32 accumulation kept below 2^24 (exact); their outputs are compared
 bit for bit.  value = complex MACs per second (Tera) of the faster one,
 algorithmic work F*T*N(N+1)/2 per step; its roofline is priced at its own
@@ -34,7 +34,7 @@ FP6_PEAK_TOPS = 10100.0  # MI355X dense FP6 matrix peak, 10.1 PFLOP/s
 MATRIX = {  # method name -> (kernel_batch, GEMM kernel, dtype, peak, plane bytes per value)
     "int8_mfma": (2, "corr_mfma_kernel", "int8 operands, int32 accumulation", INT8_PEAK_TOPS,
                   1.0),
-    "fp6_mfma": (3, "corr_f6_kernel", "fp6-e3m2 operands, fp32 accumulation (exact)",
+    "fp6_mfma": (3, "corr_f6_kernel", "fp6-e2m3 operands, fp32 accumulation (exact)",
                  FP6_PEAK_TOPS, 0.75),
 }
 
@@ -134,7 +134,9 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # The GPU clock ramps up over the first ~20 ms of work after idle; ten
+    # untimed steps let it settle before the timed ones.
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-packed", action="store_true",
                     help="skip timing the plain packed-integer paths")

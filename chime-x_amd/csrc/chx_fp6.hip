@@ -1,11 +1,16 @@
 // chx_fp6.hip -- the correlator on CDNA4 FP6 matrix cores, exactly.
 //
-// The 4-bit values v = nibble - 8 in [-8, 7] are all exact in the FP6
-// E3M2 format (1 sign, 3 exponent bits with bias 3, 2 mantissa bits):
-// 0 -> 0, 1..8 -> 2^e (1 + m/4).  v_mfma_scale_f32_32x32x64_f8f6f4 with
-// both operands E3M2 (and block scales 2^0) issues a 32x32x64 product in the
-// cycles an int8 32x32x32 one takes, and accumulates in FP32.  Per 32x32
-// block two accumulators:
+// The 4-bit values v = nibble - 8 in [-8, 7] are carried as v/8 in the FP6
+// E2M3 format (1 sign, 2 exponent bits with bias 1, 3 mantissa bits): |v| =
+// 0..7 are the subnormals m/8 (mantissa m = |v|) and |v| = 8 is 1.0
+// (exponent 1, mantissa 0), so every code is simply sign<<5 | |v|.
+// v_mfma_scale_f32_32x32x64_f8f6f4 with both operands E2M3 and a block
+// scale of 2^3 on each issues a 32x32x64 product of the integers v in the
+// cycles an int8 32x32x32 one takes, and accumulates in FP32.  (The same
+// values as E3M2 integers, all normal numbers with scales 2^0, run 5%
+// slower: on random data the matrix cores are power-limited, and the
+// subnormal operands hold a higher clock.)  Per 32x32 block two
+// accumulators:
 //   RE = Ar.Br^T + Ai.Bi^T,  IM = Ar.Bi^T + Ai.(-Br)^T,  im = U ? -IM : IM,
 // where -Br is Br with the sign bit of every code flipped (exact).  Every
 // product is an integer of magnitude <= 64, so a sample adds at most 128 to
@@ -15,7 +20,7 @@
 // path's.
 //
 // Two passes per time chunk:
-//  1. unpack_f6_kernel (HBM-bound): [time][freq][element] bytes -> E3M2
+//  1. unpack_f6_kernel (HBM-bound): [time][freq][element] bytes -> E2M3
 //     codes, bit-packed 32 per 24 bytes (code j at bits 6j..6j+5, little
 //     endian), as two arrays per plane: P0 = bytes 0-15 and P1 = bytes 16-23
 //     of each element's 32-sample half of a 64-sample block:
@@ -58,13 +63,19 @@ constexpr int kSlotBytes = kP0Bytes + kP1Bytes;  // one 128-row block of one sta
 constexpr int kRingBytes = kRing * kSlotBytes;   // a ring of one row block
 static_assert(kMaxStages % kRing == 0, "work items hold whole rings");
 
-// Four nibbles (one per byte, 0-15) -> the E3M2 codes of nibble - 8 (one per
-// byte), by a 16-entry byte table in four words: entries 0-7 (v = -8..-1)
-// in the first pair, 8-15 (v = 0..7) in the second.
-__device__ __forceinline__ uint32_t e3m2_codes(uint32_t n) {
+// MFMA operand format and block scale: E2M3 (format 2 of the f8f6f4
+// instruction) holding v/8, scaled back by the E8M0 exponent 127 + 3.
+constexpr int kFmtE2M3 = 2;
+constexpr int kScale2p3 = 127 + 3;
+
+// Four nibbles (one per byte, 0-15) -> the E2M3 codes of (nibble - 8) / 8
+// (one per byte), by a 16-entry byte table in four words: entries 0-7
+// (v = -8..-1: sign | 8..1) in the first pair, 8-15 (v = 0..7) in the
+// second.
+__device__ __forceinline__ uint32_t fp6_codes(uint32_t n) {
   const uint32_t sel = n & 0x07070707u;
-  const uint32_t neg = __builtin_amdgcn_perm(0x2C303234u, 0x35363738u, sel);
-  const uint32_t pos = __builtin_amdgcn_perm(0x17161514u, 0x12100C00u, sel);
+  const uint32_t neg = __builtin_amdgcn_perm(0x21222324u, 0x25262728u, sel);
+  const uint32_t pos = __builtin_amdgcn_perm(0x07060504u, 0x03020100u, sel);
   const uint32_t m = ((n >> 3) & 0x01010101u) * 0xFFu;  // 0xFF where nibble >= 8
   return (pos & m) | (neg & ~m);
 }
@@ -149,7 +160,7 @@ __device__ __forceinline__ void unpack_f6_unit(const uint8_t *__restrict__ in,
         uint32_t code[8];
 #pragma unroll
         for (int m = 0; m < 8; m++)
-          code[m] = e3m2_codes(pl == 0 ? (c[e][m] >> 4) & 0x0F0F0F0Fu : c[e][m] & 0x0F0F0F0Fu);
+          code[m] = fp6_codes(pl == 0 ? (c[e][m] >> 4) & 0x0F0F0F0Fu : c[e][m] & 0x0F0F0F0Fu);
         const int el = el0 + e;
         const int slot = el + (el >> 4);  // one pad entry per 16
         v4u a;
@@ -441,17 +452,17 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
       const v8i xr = AR[p][i], xi = AI[p][i];
 #pragma unroll
       for (int j = 0; j < 2; j++) {
-        re[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(xr, br[j], re[i][j], 3, 3,
-                                                                   0, 127, 0, 127);
-        im[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(xr, bi[j], im[i][j], 3, 3,
-                                                                   0, 127, 0, 127);
+        re[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+            xr, br[j], re[i][j], kFmtE2M3, kFmtE2M3, 0, kScale2p3, 0, kScale2p3);
+        im[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+            xr, bi[j], im[i][j], kFmtE2M3, kFmtE2M3, 0, kScale2p3, 0, kScale2p3);
       }
 #pragma unroll
       for (int j = 0; j < 2; j++) {
-        re[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(xi, bi[j], re[i][j], 3, 3,
-                                                                   0, 127, 0, 127);
-        im[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(xi, nbr[j], im[i][j], 3, 3,
-                                                                   0, 127, 0, 127);
+        re[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+            xi, bi[j], re[i][j], kFmtE2M3, kFmtE2M3, 0, kScale2p3, 0, kScale2p3);
+        im[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+            xi, nbr[j], im[i][j], kFmtE2M3, kFmtE2M3, 0, kScale2p3, 0, kScale2p3);
       }
       AR[p ^ 1][i] = load_op(nslot, 0, i);
       AI[p ^ 1][i] = load_op(nslot, 1, i);

@@ -63,7 +63,7 @@ size_t mfma_workspace_bytes(const chx_plan *p);
 int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis,
                      int mode, hipStream_t s);
 
-// FP6 (E3M2) matrix-core path (chx_fp6.hip): same modes, bit-identical
+// FP6 (E2M3) matrix-core path (chx_fp6.hip): same modes, bit-identical
 // results.
 constexpr int kFp6Ring = 4;  // 64-sample stages in the LDS ring
 // Both matrix-core GEMMs use output tiles of 128 rows x 256 columns: row

@@ -33,8 +33,9 @@ extern "C" {
  * (main_wrapper.c:281-305); 2 and 3 have no reference counterpart: the
  * matrix-core paths, which need no offset/preseed pass and have no
  * time_accum cap.  2 computes on int8 operands with int32 accumulation; 3 on
- * FP6 E3M2 operands (every 4-bit value is exact in E3M2) with FP32
- * accumulation kept below 2^24, so its results are bit-identical to 2's. */
+ * FP6 E2M3 operands (every 4-bit value v is exact as v/8, scaled back by a
+ * block scale of 2^3) with FP32 accumulation kept below 2^24, so its results
+ * are bit-identical to 2's. */
 #define CHX_METHOD_CONFERENCE 0
 #define CHX_METHOD_HIGHLY_PACKED 1
 #define CHX_METHOD_INT8_MFMA 2
