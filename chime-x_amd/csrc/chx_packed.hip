@@ -46,32 +46,54 @@ __global__ void fill_u32_kernel(uint32_t *p, uint32_t v, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------
-// offsetAccumulateElements.  Thread = 16 consecutive bytes of a time row
-// (elements of one or more channels; the row is F*N bytes), over `tc`
-// samples.  Nibbles are summed in 16-bit lanes (even bytes / odd bytes of
-// each word in separate registers): 15 * tc <= 65535 for tc <= 4096.  The
-// totals are added into sums with atomics, as the reference does.
+// offsetAccumulateElements.  Workgroup = 64 columns of 16 consecutive bytes
+// of a time row (elements of one or more channels; the row is F*N bytes)
+// over `tc` samples; its 4 waves take every 4th row (wave w: rows w, w+4,
+// ...), so together they read 4 whole 1 KB row pieces at a time.  Nibbles
+// are summed in 16-bit lanes (even bytes / odd bytes of each word in
+// separate registers); the waves' lanes are added through LDS, 15 * tc <=
+// 65535 in all for tc <= 4096, and wave 0 adds the totals into sums with
+// atomics, as the reference does (a quarter of the atomics of one thread
+// per column and slice).
 __global__ __launch_bounds__(256) void offset_accumulate_kernel(
     const v4u *__restrict__ in, uint32_t *__restrict__ sums, int64_t row_u4,
     int64_t T, int tc) {
-  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= row_u4) return;
+  __shared__ uint32_t part[3][16][64];  // waves 1-3: [wave - 1][accumulator][lane]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t col = (int64_t)blockIdx.x * 64 + lane;
+  const bool valid = col < row_u4;
   const int64_t t_begin = (int64_t)blockIdx.y * tc;
   const int64_t t_end = min(T, t_begin + tc);
   uint32_t acc[4][4] = {};  // [word][R even, R odd, I even, I odd]
-  const v4u *p = in + t_begin * row_u4 + col;
+  if (valid) {
+    const v4u *p = in + (t_begin + wave) * row_u4 + col;
 #pragma unroll 4
-  for (int64_t t = t_begin; t < t_end; t++, p += row_u4) {
-    const v4u v = __builtin_nontemporal_load(p);
-    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    for (int64_t t = t_begin + wave; t < t_end; t += 4, p += 4 * row_u4) {
+      const v4u v = __builtin_nontemporal_load(p);
+      const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      acc[q][0] += (w[q] >> 4) & 0x000F000Fu;
-      acc[q][1] += (w[q] >> 12) & 0x000F000Fu;
-      acc[q][2] += w[q] & 0x000F000Fu;
-      acc[q][3] += (w[q] >> 8) & 0x000F000Fu;
+      for (int q = 0; q < 4; q++) {
+        acc[q][0] += (w[q] >> 4) & 0x000F000Fu;
+        acc[q][1] += (w[q] >> 12) & 0x000F000Fu;
+        acc[q][2] += w[q] & 0x000F000Fu;
+        acc[q][3] += (w[q] >> 8) & 0x000F000Fu;
+      }
     }
   }
+  if (wave > 0) {
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) part[wave - 1][4 * q + k][lane] = acc[q][k];
+  }
+  __syncthreads();
+  if (wave != 0 || !valid) return;
+#pragma unroll
+  for (int w = 0; w < 3; w++)
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) acc[q][k] += part[w][4 * q + k][lane];
   uint32_t *o = sums + col * 32;
 #pragma unroll
   for (int q = 0; q < 4; q++) {
@@ -407,7 +429,7 @@ int launch_offset_accumulate(const chx_plan *p, const uint8_t *in, uint32_t *sum
                              hipStream_t s) {
   const int64_t T = p->cfg.time_steps;
   const int64_t row_u4 = (int64_t)p->cfg.num_freq * p->cfg.num_elements / 16;
-  const int64_t gx = (row_u4 + 255) / 256;
+  const int64_t gx = (row_u4 + 63) / 64;
   // Enough slices to fill the chip with ~4096 wavefronts, at most 4096
   // samples per slice (16-bit lane bound), at least 32.
   int64_t tc = (T * gx * 4) / 4096;
