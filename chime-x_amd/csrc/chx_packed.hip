@@ -304,21 +304,24 @@ __global__ __launch_bounds__(64) void corr_highly_packed_kernel(
       const uint32_t wx[2] = {vx.x, vx.y};
 #pragma unroll
       for (int q = 0; q < 2; q++) {
+        // nibbles one per byte; v_perm_b32 then builds each lane value
+        // (selector 0-3: bytes of the low nibbles, 4-7: the high, 12: zero)
+        const uint32_t ylo = wy[q] & 0x0F0F0F0Fu, yhi = (wy[q] >> 4) & 0x0F0F0F0Fu;
+        const uint32_t xlo = wx[q] & 0x0F0F0F0Fu, xhi = (wx[q] >> 4) & 0x0F0F0F0Fu;
         uint32_t pyv[4], ryv[4], pxv[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-          const uint32_t by8 = (wy[q] >> (8 * k)) & 0xFFu;
-          const uint32_t bx8 = (wx[q] >> (8 * k)) & 0xFFu;
-          pyv[k] = ((by8 >> 4) << 16) | (by8 & 15u);
-          ryv[k] = by8 >> 4;
-          pxv[k] = ((bx8 >> 4) << 16) | (bx8 & 15u);
+          const uint32_t pk_sel = 0x0C000C00u | ((4u + k) << 16) | k;  // (hi<<16) | lo
+          pyv[k] = __builtin_amdgcn_perm(yhi, ylo, pk_sel);
+          ryv[k] = __builtin_amdgcn_perm(yhi, ylo, 0x0C0C0C00u | (4u + k));
+          pxv[k] = __builtin_amdgcn_perm(xhi, xlo, pk_sel);
         }
         *(uint4 *)&hy[ls][8 * lq + 4 * q] = make_uint4(pyv[0], pyv[1], pyv[2], pyv[3]);
         *(uint4 *)&hry[ls][8 * lq + 4 * q] = make_uint4(ryv[0], ryv[1], ryv[2], ryv[3]);
         *(uint4 *)&hx[ls][8 * lq + 4 * q] = make_uint4(pxv[0], pxv[1], pxv[2], pxv[3]);
-        *(uint2 *)&hrx[ls][4 * lq + 2 * q] =
-            make_uint2((pxv[0] & 0x000F0000u) | (pxv[1] >> 16),
-                       (pxv[2] & 0x000F0000u) | (pxv[3] >> 16));
+        *(uint2 *)&hrx[ls][4 * lq + 2 * q] =  // (R_x0<<16) | R_x1, (R_x2<<16) | R_x3
+            make_uint2(__builtin_amdgcn_perm(xhi, xlo, 0x0C040C05u),
+                       __builtin_amdgcn_perm(xhi, xlo, 0x0C060C07u));
       }
     }
     __syncthreads();
