@@ -232,6 +232,23 @@ def test_paired_16_feed_run_against_oracle(method, U):
     np.testing.assert_array_equal(tri, O.correlate_triangle(data, U))
 
 
+@pytest.mark.parametrize("T,F,N", [(4192, 3, 416), (65632, 1, 32), (2048, 16, 2048)])
+def test_offset_sums_ragged_columns_and_slices(T, F, N):
+    # column groups of 64 x 16 bytes with a partial last one (F*N/16 = 78),
+    # many slices with a ragged last one (T = 65536 + 96), and a full row of
+    # the array-scale shape
+    torch = _torch()
+    data = rand(T, F, N, 5)
+    say(f"  offset sums T={T} F={F} N={N}")
+    with chx.Plan(N, F, T, 32, 1, chx.CONFERENCE) as p:
+        d_in = torch.from_numpy(data).cuda()
+        d_sums = torch.zeros((F, N, 2), dtype=torch.int32, device="cuda")
+        p.offset_accumulate(d_in.data_ptr(), d_sums.data_ptr(),
+                            torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(d_sums.cpu().numpy().view(np.uint32), O.offset_sums(data))
+
+
 def test_full_matrix_expansion_on_device(golden):
     torch = _torch()
     for N, F in ((96, 2), (64, 1)):
