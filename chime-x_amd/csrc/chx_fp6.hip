@@ -250,7 +250,7 @@ __device__ __forceinline__ void flush_block_f32(int32_t *blk, const v16f &re, co
   const int voff = ((lane >> 5) * 4 * 32 + (lane & 31)) * 8;
 #pragma unroll
   for (int j = 0; j < 16; j++) {
-    const int32_t vr = (int32_t)re[j];  // |.| < 2^23: exact
+    const int32_t vr = (int32_t)re[j];  // an integer of magnitude <= 2^24: exact
     const int32_t vi = (int32_t)(U ? -im[j] : im[j]);
     const int soff = (8 * (j >> 2) + (j & 3)) * 32 * 8;
     if (mode == 0) {

@@ -212,3 +212,38 @@ def test_fp6_item_at_the_exact_bound():
         torch.cuda.synchronize()
     v = vis.cpu().numpy()
     assert (v[..., 0] == 128 * T).all() and (v[..., 1] == 0).all()
+
+
+def test_fp6_item_near_the_bound_with_mixed_signs():
+    """A whole-chunk FP6 item of 2048 stages whose accumulators run near
+    2^24 with products of both signs: for the first 2047 stages, elements of
+    a flip set alternate per sample between (-8,-8) and (7,7) (products
+    +128 / -112 against the others, +128 / +98 among themselves) and the
+    rest hold (-8,-8) (+128 per sample); the last stage and the one-ring
+    rest are uniform random.  FP6 equals the int8 method and the oracle."""
+    torch = _torch()
+    T, F, N = 2048 * 64 + 256, 1, 128
+    t_const = 2047 * 64
+    data = np.random.default_rng(24).integers(0, 256, size=(T, F, N), dtype=np.uint8)
+    data[:t_const] = 0x00
+    flip = np.random.default_rng(25).choice(N, size=N // 3, replace=False)
+    odd = np.arange(t_const) % 2 == 1
+    data[np.ix_(np.nonzero(odd)[0], [0], flip)] = 0xFF
+    want = O.correlate_blocks(data, 1)
+    assert np.abs(want[..., 0]).max() > (1 << 24) - 8192  # near the cap
+    x = torch.from_numpy(data).cuda()
+    got = {}
+    for method in (chx.FP6_MFMA, chx.INT8_MFMA):
+        vis = torch.full((F, chx.num_blocks(N), 32, 32, 2), 3, dtype=torch.int32,
+                         device="cuda")
+        job = dict(num_elements=N, num_freq=F, time_steps=T, upper_triangle_convention=1,
+                   d_input=x.data_ptr(), d_vis=vis.data_ptr())
+        with chx.Batch([job], dict(method=method, workspace_bytes=1 << 30)) as b:
+            inf = b.info()
+            say(f"  near-bound batch {chx.METHOD_NAMES[method]}: {inf}")
+            assert inf["rounds"] == 1 and inf["items"] == F * 1 * 2
+            b.run()
+            torch.cuda.synchronize()
+        got[method] = vis.cpu().numpy()
+    np.testing.assert_array_equal(got[chx.FP6_MFMA], got[chx.INT8_MFMA])
+    np.testing.assert_array_equal(got[chx.FP6_MFMA], want)

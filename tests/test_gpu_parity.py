@@ -275,3 +275,130 @@ def test_full_matrix_expansion_on_device(golden):
         full = d_full.cpu().numpy()
         np.testing.assert_array_equal(full[..., 0], full[..., 0].transpose(0, 2, 1))
         np.testing.assert_array_equal(full[..., 1], -full[..., 1].transpose(0, 2, 1))
+
+
+# ---- Config 2 (N=2048, F=16) at the launch shapes the headline runs ----
+
+CFG2_T, CFG2_F, CFG2_N = 65536, 16, 2048
+
+
+def _launch_shape(F, N, T, cus):
+    """The matrix-core plan launch for one chunk (both methods): whole rounds
+    of one workgroup per CU take whole work items; the items of the last
+    partial round are cut in time into `split` pieces that add atomically
+    into output zeroed beforehand (the channels from head // items-per-channel
+    on)."""
+    nt = (N + 127) // 128
+    m = nt // 2
+    wide = (m + 1) * (m + 1) if nt % 2 else m * (m + 1)
+    per_f = wide - (nt + 1) // 2 // 2
+    items = F * per_f
+    nst = -(-T // 256) * 4
+    head = items // cus * cus if nst <= 2048 else 0
+    tail = items - head
+    split = 1
+    if tail:
+        best = -(-tail // cus)
+        for sp in range(2, 65):
+            if nst // sp < 16:
+                break
+            r = -(-(tail * sp) // cus) / sp
+            if r < best - 1e-9:
+                best, split = r, sp
+    sps = min(-(-(-(-nst // split)) // 4) * 4, 2048)
+    split = -(-nst // sps)
+    return per_f, head, tail, split
+
+
+def _oracle_blocks_by_channel(data, U):
+    """The oracle's block buffer, one channel per thread (channels are
+    independent in the reference loop, cpu_corr_test.c:176-198)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    T, F, N = data.shape
+    out = np.empty((F, chx.num_blocks(N), 32, 32, 2), np.int32)
+
+    def one(f):
+        out[f] = O.correlate_blocks(np.ascontiguousarray(data[:, f:f + 1, :]), U)[0]
+
+    with ThreadPoolExecutor(max_workers=16) as ex:
+        list(ex.map(one, range(F)))
+    return out
+
+
+@pytest.fixture(scope="module")
+def cfg2_slice_oracle():
+    T = 2048
+    data = rand(T, CFG2_F, CFG2_N, 22)
+    say(f"  oracle T={T} F={CFG2_F} N={CFG2_N} (threads over channels)")
+    return data, _oracle_blocks_by_channel(data, 1)
+
+
+@pytest.mark.parametrize("method", MATRIX)
+def test_config2_head_and_tail_pieces_against_oracle(method, cfg2_slice_oracle):
+    # N=2048, F=16, T=2048 takes the headline launch: whole items with plain
+    # stores for the whole rounds, then the last partial round's items cut
+    # into time pieces that add atomically after a zero fill of the last
+    # channel; checked against the oracle's full block buffer.
+    torch = _torch()
+    data, want = cfg2_slice_oracle
+    T = data.shape[0]
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    per_f, head, tail, split = _launch_shape(CFG2_F, CFG2_N, T, cus)
+    say(f"  launch: {cus} CUs, {per_f} items per channel, head {head}, tail {tail}, "
+        f"split {split}, zero fill from channel {head // per_f}")
+    assert head > 0 and tail > 0 and split > 1, "shape must take whole items and tail pieces"
+    assert head // per_f < CFG2_F
+    # output starts non-zero, so the fill and the plain stores both matter
+    with chx.Plan(CFG2_N, CFG2_F, T, 256, 1, method) as p:
+        d_in = torch.from_numpy(data).cuda()
+        d_vis = torch.full(p.output_shape, -5, dtype=torch.int32, device="cuda")
+        p.correlate_device(d_in.data_ptr(), d_vis.data_ptr())
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(d_vis.cpu().numpy(), want)
+
+
+def test_config2_full_size_all_methods():
+    # Config 2 at full size (2 GiB of input, 16 x 2,098,176 visibilities):
+    # all four methods bit-equal, the conjugate convention the conjugate, and
+    # every channel equal to the oracle on a sorted 16-element subset over
+    # all 65,536 samples (elements of the first and the last tile included).
+    torch = _torch()
+    T, F, N = CFG2_T, CFG2_F, CFG2_N
+    x = torch.empty((T, F, N), dtype=torch.uint8, device="cuda")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2)
+    torch.randint(0, 256, x.shape, dtype=torch.uint8, device="cuda", generator=g, out=x)
+    outs = {}
+    for method, ta in ((chx.FP6_MFMA, 256), (chx.INT8_MFMA, 256), (chx.CONFERENCE, 256),
+                       (chx.HIGHLY_PACKED, 2048)):
+        say(f"  config 2 full size: {chx.METHOD_NAMES[method]}")
+        with chx.Plan(N, F, T, ta, 1, method) as p:
+            out = torch.empty(p.output_shape, dtype=torch.int32, device="cuda")
+            p.correlate_device(x.data_ptr(), out.data_ptr())
+            torch.cuda.synchronize()
+        outs[method] = out
+    ref = outs[chx.FP6_MFMA]
+    for method, out in outs.items():
+        assert torch.equal(out, ref), chx.METHOD_NAMES[method]
+    with chx.Plan(N, F, T, 256, 0, chx.INT8_MFMA) as p:
+        conj = torch.empty_like(ref)
+        p.correlate_device(x.data_ptr(), conj.data_ptr())
+        torch.cuda.synchronize()
+    assert torch.equal(conj[..., 0], ref[..., 0])
+    assert torch.equal(conj[..., 1], -ref[..., 1])
+    del conj, outs
+    rng = np.random.default_rng(65536)
+    fixed = [0, 37, 1930, 2047]
+    rest = rng.choice(np.setdiff1d(np.arange(N), fixed), size=12, replace=False)
+    E = np.sort(np.concatenate([fixed, rest]))
+    sub = x[:, :, torch.as_tensor(E, device="cuda")].contiguous().cpu().numpy()  # [T][F][16]
+    want = O.correlate_triangle(sub, 1)  # [F][136][2]
+    ys, xs = np.triu_indices(16)
+    Y, X = E[ys], E[xs]
+    nb = N // 32
+    by, bx = Y // 32, X // 32
+    b = by * nb - by * (by - 1) // 2 + (bx - by)
+    got = ref[:, torch.as_tensor(b, device="cuda"), torch.as_tensor(Y % 32, device="cuda"),
+              torch.as_tensor(X % 32, device="cuda")].cpu().numpy()
+    np.testing.assert_array_equal(got, want)
