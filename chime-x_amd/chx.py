@@ -45,7 +45,7 @@ class ChxError(RuntimeError):
         self.status = status
         detail = lib().chx_last_error_detail().decode()
         msg = f"{where}: {lib().chx_strerror(status).decode()} (status {status})"
-        if detail and status == ERR_HIP:
+        if detail and (status == ERR_HIP or where == "chx_correlate_noalloc"):
             msg += f" [{detail}]"
         super().__init__(msg)
 
@@ -152,6 +152,7 @@ def lib() -> ctypes.CDLL:
     for name in ("chx_offset_accumulate", "chx_preseed", "chx_corr", "chx_correlate"):
         getattr(L, name).argtypes = [vp, vp, vp, vp]
     L.chx_correlate_host.argtypes = [vp, vp, vp]
+    L.chx_correlate_noalloc.argtypes = [ctypes.POINTER(Config), vp, vp, vp]
     L.chx_plan_profile.argtypes = [vp, i32]
     L.chx_plan_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double),
@@ -351,6 +352,21 @@ class Plan:
         _check(lib().chx_plan_kernel_times(self._h, ctypes.byref(u), ctypes.byref(g),
                                            ctypes.byref(n)), "chx_plan_kernel_times")
         return u.value, g.value, n.value
+
+
+NOALLOC_SCRATCH_BYTES = 2 << 20  # CHX_NOALLOC_SCRATCH_BYTES
+
+
+def correlate_noalloc(num_elements: int, num_freq: int, time_steps: int, d_input: int,
+                      d_vis: int, time_accum: int = 256, upper_triangle_convention: int = 1,
+                      kernel_batch: int = CONFERENCE, device: int = 0, stream: int = 0) -> None:
+    """chx_correlate_noalloc: one correlation of device input into device
+    output with no plan and no allocation (packed methods only; scratch in a
+    fixed per-GPU device array shared by every call on that GPU)."""
+    cfg = Config(num_elements, num_freq, time_steps, time_accum, upper_triangle_convention,
+                 kernel_batch, device)
+    _check(lib().chx_correlate_noalloc(ctypes.byref(cfg), d_input, d_vis, stream or None),
+           "chx_correlate_noalloc")
 
 
 def blocks_to_triangle_device(num_freq: int, N: int, d_blocks: int, d_tri: int,

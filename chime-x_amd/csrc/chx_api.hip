@@ -21,9 +21,13 @@ int hip_status(hipError_t e) {
   return CHX_ERR_HIP;
 }
 
+void set_error_detail(const char *msg) {
+  snprintf(g_last_error, sizeof(g_last_error), "%s", msg);
+}
+
 // Shape rules (main_wrapper.c:277-278, 564-572 and the kernels' loop
 // steps), enforced here because the reference does not check them.
-static int validate(const chx_config *c) {
+int validate_config(const chx_config *c) {
   if (!c) return CHX_ERR_INVALID_ARGUMENT;
   if (c->kernel_batch < CHX_METHOD_CONFERENCE || c->kernel_batch > CHX_METHOD_FP6_MFMA)
     return CHX_ERR_INVALID_ARGUMENT;
@@ -211,7 +215,7 @@ int chx_blocks_to_triangle_device(int32_t num_freq, int32_t num_elements,
 int chx_plan_create(const chx_config *cfg, chx_plan **out) {
   if (!out) return CHX_ERR_INVALID_ARGUMENT;
   *out = nullptr;
-  int rc = validate(cfg);
+  int rc = validate_config(cfg);
   if (rc) return rc;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return CHX_ERR_NO_DEVICE;

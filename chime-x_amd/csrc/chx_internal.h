@@ -31,6 +31,10 @@ namespace chx {
 
 // Records a HIP error for chx_last_error_detail() and maps it to a status.
 int hip_status(hipError_t e);
+// Sets the text chx_last_error_detail() returns on this thread.
+void set_error_detail(const char *msg);
+// The shape and time_accum rules of a chx_config (a chx_status).
+int validate_config(const chx_config *c);
 
 #define CHX_HIP(expr)                              \
   do {                                             \

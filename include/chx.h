@@ -123,6 +123,21 @@ int chx_correlate(chx_plan *plan, const uint8_t *d_input, int32_t *d_vis,
  * buffer out; synchronous.  Device buffers are owned by the plan. */
 int chx_correlate_host(chx_plan *plan, const uint8_t *h_input, int32_t *h_vis);
 
+/* chx_correlate without a plan and without any allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Methods 0 and 1 only (the matrix-core methods need a plane
+ * workspace: CHX_ERR_INVALID_ARGUMENT).  Their scratch -- the block id maps
+ * and the offset sums, (2 NB rounded up to 64 + 2 F N) uint32 -- lives in a
+ * static device array of CHX_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; a larger request returns CHX_ERR_BUDGET (chx_last_error_detail
+ * says how much it needs).  Each call rewrites the scratch before use, but
+ * all calls on one GPU share it: calls that overlap on one GPU (other
+ * streams, other threads) race.  Assigns d_vis like chx_correlate. */
+#define CHX_NOALLOC_SCRATCH_BYTES (2u << 20)
+int chx_correlate_noalloc(const chx_config *cfg, const uint8_t *d_input, int32_t *d_vis,
+                          void *stream);
+
 /* Kernel timing (no reference counterpart; for roofline measurements):
  * while enabled, every matrix-core chunk (int8 or FP6) the plan launches records HIP
  * events on its stream before the unpack kernel, between it and the GEMM

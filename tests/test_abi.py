@@ -102,6 +102,27 @@ def test_valid_config_without_gpu_reports_no_device():
     assert e.value.status == chx.ERR_NO_DEVICE
 
 
+def test_noalloc_refusals_are_host_side():
+    # chx_correlate_noalloc refuses a request its fixed scratch cannot hold
+    # and the matrix-core methods before touching a device (the pointers are
+    # never used).
+    fake = 1 << 40
+    with pytest.raises(chx.ChxError) as e:
+        chx.correlate_noalloc(4096, 64, 256, fake, fake)  # 2 MiB of sums + the maps
+    assert e.value.status == chx.ERR_BUDGET and "limit" in str(e.value)
+    with pytest.raises(chx.ChxError) as e:
+        chx.correlate_noalloc(256, 1, 256, fake, fake, kernel_batch=chx.FP6_MFMA)
+    assert e.value.status == chx.ERR_INVALID_ARGUMENT and "plan" in str(e.value)
+    with pytest.raises(chx.ChxError) as e:
+        chx.correlate_noalloc(256, 1, 250, fake, fake)
+    assert e.value.status == chx.ERR_SHAPE
+    # the largest request that fits: 4096 x 32 (config 5's array)
+    if chx.device_count() == 0:
+        with pytest.raises(chx.ChxError) as e:
+            chx.correlate_noalloc(4096, 32, 256, fake, fake)
+        assert e.value.status == chx.ERR_NO_DEVICE
+
+
 def _batch_create(jobs, tuning=None, n=None):
     arr = (chx.JobDesc * max(1, len(jobs)))()
     for k, j in enumerate(jobs):
