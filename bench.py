@@ -15,6 +15,12 @@ algorithmic work F*T*N(N+1)/2 per step; its roofline is priced at its own
 dense peak (int8 5.0 POP/s, FP6 10.1 PFLOP/s).  The plain packed-integer
 paths (conference, highly packed) are timed beside them, and the conference
 output is compared bit for bit too.
+
+Two more workloads are reported beside the line (not in `value`): the uneven
+batch of 64 jobs (N in {256..2048}, 1-8 channels, 4096-131072 samples) as
+one captured graph replay with its cached tuning, and the streaming engine
+(N=2048, F=8, 1024-sample frames, device-resident) saturated and at a
+Poisson load equal to the saturated rate.
 """
 from __future__ import annotations
 
@@ -41,6 +47,22 @@ MATRIX = {  # method name -> (kernel_batch, GEMM kernel, dtype, peak, plane byte
 
 def cmac(F, T, N):
     return F * T * N * (N + 1) // 2
+
+
+def cpu_info() -> dict:
+    """Host cores: online count, the ones this process may use, the model."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = (len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+              else (os.cpu_count() or 1))
+    return {"nproc": os.cpu_count(), "usable": usable, "model": model}
 
 
 def cpu_baseline(budget_s: float = 12.0):
@@ -82,8 +104,8 @@ def cpu_baseline(budget_s: float = 12.0):
     one = cmac(1, T, N) / dt / 1e12
     # one channel per thread (ctypes releases the GIL during the call)
     import threading
-    cores = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
-                       else (os.cpu_count() or 1)))
+    info = cpu_info()
+    cores = max(1, min(16, info["usable"]))
     outs = [(ctypes.c_int32 * (N * (N + 1)))() for _ in range(cores)]
     th = [threading.Thread(target=call, args=(T, outs[k])) for k in range(cores)]
     t0 = time.perf_counter()
@@ -98,23 +120,30 @@ def cpu_baseline(budget_s: float = 12.0):
         "cores": cores,
         "kind": kind,
         "single_thread_value": one,
+        "host": info,
         "sample": f"N={N}, one channel of the workload per thread, T={T} continuous seeded "
                   f"random, generator included: 1 thread {dt:.2f} s, {cores} threads "
                   f"{dtm:.2f} s wall",
+        "note": "threads capped at 16 (the host share of one GPU); the reference loop "
+                "(t outer, cpu_corr_test.c:176-198) sweeps the whole N(N+1)/2 int32 "
+                "triangle (16.8 MB at N=2048) once per sample on every thread, so the "
+                "threads share the memory bandwidth and scale sub-linearly",
     }
 
 
-def traffic_from_profile(kernel: str):
-    """HBM bytes per GEMM launch measured with rocprofv3 PMC counters
-    (FETCH_SIZE + WRITE_SIZE, each scaled by the same counters' reading for a
-    copy of known size), as recorded in the kernel's profile summary."""
+def from_profile(kernel: str, key: str):
+    """A figure recorded in the kernel's rocprofv3 summary under profiles/:
+    traffic_bytes_per_launch (HBM bytes per GEMM launch from PMC FETCH_SIZE +
+    WRITE_SIZE, each scaled by the same counters' reading for a copy of known
+    size) or mfma_busy_pct (MfmaUtil: SQ_VALU_MFMA_BUSY_CYCLES over
+    GRBM_GUI_ACTIVE x SIMDs, steady launches)."""
     import re
     path = os.path.join(HERE, "profiles", f"{kernel.replace('_kernel', '')}_N2048_F16_T65536.md")
     try:
         text = open(path).read()
     except OSError:
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    m = re.search(key + r":\s*([0-9.eE+]+)", text)
     return float(m.group(1)) if m else None
 
 
@@ -130,6 +159,73 @@ def copy_calibration(torch, gib: int = 1, reps: int = 5):
                       "expected_fetch_bytes": n, "expected_write_bytes": n}))
 
 
+def batch_uneven64(torch, chx, reps: int = 5) -> dict:
+    """The 64-job uneven batch as one graph replay, with its cached tuning."""
+    import workloads as W
+    jobs = W.uneven_batch_jobs()
+    ins, outs, descs = [], [], []
+    for j in jobs:
+        N, F, T = j["num_elements"], j["num_freq"], j["time_steps"]
+        ins.append(W.device_uniform(torch, (T, F, N), j["seed"]))
+        outs.append(torch.empty((F, chx.num_blocks(N), 32, 32, 2), dtype=torch.int32,
+                                device="cuda"))
+        descs.append(dict(j, d_input=ins[-1].data_ptr(), d_vis=outs[-1].data_ptr()))
+    total = sum(W.cmacs(j["num_elements"], j["num_freq"], j["time_steps"]) for j in jobs)
+    tuning = chx.cached_tuning(jobs)
+    stream = torch.cuda.current_stream()
+    with chx.Batch(descs, tuning) as b:
+        b.run(stream=stream.cuda_stream)  # capture + first replay
+        b.run(stream=stream.cuda_stream)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            b.run(stream=stream.cuda_stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        info = b.info()
+    del ins, outs
+    torch.cuda.empty_cache()
+    ms = sorted(ts)[len(ts) // 2]
+    return {"workload": "64 jobs, N in {256,512,1024,2048}, 1-8 channels, 4096-131072 "
+                        "samples (seed 3), one captured graph replay",
+            "value": round(total / ms / 1e9, 2), "unit": "TCMAC/s", "ms_per_replay": round(ms, 3),
+            "replays_timed": reps, "cmacs": total, "tuning": tuning,
+            "rounds": info["rounds"], "items": info["items"]}
+
+
+def serve_streaming(torch, chx, frames: int = 1024) -> dict:
+    """Streaming engine, N=2048, F=8, 1024-sample frames from device memory
+    (5% of the elements held at (7,-8)): the saturated rate, then Poisson
+    arrivals at that rate (1x load) with p50/p99 frame-to-visibility latency."""
+    import serve_driver as D
+    import workloads as W
+    N, F, T = 2048, 8, 1024
+    fb = T * F * N
+    data = W.streaming_frame(torch, frames * T, F, N, seed=4)
+    torch.cuda.synchronize()
+    ptrs = [data.data_ptr() + i * fb for i in range(frames)]
+    with chx.Serve(frame_slots=64, slot_bytes=fb) as sv:
+        sid = sv.open(N, F, T)
+        sat = D.saturated(sv, sid, ptrs, True)
+        one = D.run_arrivals(sv, sid, ptrs,
+                             D.poisson_arrivals(frames, sat["achieved_rate"], seed=11), True)
+        sv.close_stream(sid)
+    del data
+    torch.cuda.empty_cache()
+    return {"workload": f"N={N}, F={F}, {frames} frames of {T} samples (2^20 samples), "
+                        "device-resident frames, 5% of elements held at (7,-8)",
+            "saturated_frames_per_s": round(sat["achieved_rate"], 1),
+            "saturated_tcmacs": round(sat["tcmacs"], 2),
+            "poisson_1x": {"offered_frames_per_s": round(one["offered_rate"], 1),
+                           "achieved_frames_per_s": round(one["achieved_rate"], 1),
+                           "p50_ms": round(one["p50_ms"], 3), "p99_ms": round(one["p99_ms"], 3),
+                           "mean_batch_frames": round(one["mean_batch_frames"], 2)}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +238,8 @@ def main() -> int:
                     help="skip timing the plain packed-integer paths")
     ap.add_argument("--only", choices=["int8_mfma", "fp6_mfma", "conference", "highly_packed"],
                     help="time one path only (for profiling)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the uneven-batch and streaming-engine figures")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -222,7 +320,10 @@ def main() -> int:
                 "ops": "8 * F*T*N(N+1)/2 per launch (1 MAC = 2 ops, 4 real MACs per complex "
                        "MAC), over the kernel's HIP-event duration",
                 "frac": round(achieved / peak, 4), "kernel_ms": round(gemm_ms, 4),
-                "launches_timed": launches, "traffic": traffic_from_profile(kernel),
+                "launches_timed": launches,
+                "traffic": from_profile(kernel, "traffic_bytes_per_launch"),
+                "mfma_busy_pct": from_profile(kernel, "mfma_busy_pct"),
+                "clock_ghz": from_profile(kernel, "clock_ghz"),
             },
             "unpack_kernel": {
                 "bound": "hbm", "ms": round(unpack_ms, 4),
@@ -254,6 +355,13 @@ def main() -> int:
     if not mat:
         print(json.dumps({"only": only, "packed": packed}))
         return 0
+
+    extras = {}
+    if not args.no_extras and only is None:
+        del outs, ref_out, d_in
+        torch.cuda.empty_cache()
+        extras["batch_uneven64"] = batch_uneven64(torch, chx)
+        extras["serve_streaming"] = serve_streaming(torch, chx)
 
     best = max(mat, key=lambda n: mat[n]["value"])
     b = mat[best]
@@ -288,6 +396,7 @@ def main() -> int:
         "speedup_vs_conference": (round(b["value"] / packed["conference"]["value"], 2)
                                   if "conference" in packed else None),
         "cpu_baseline": cpu,
+        **extras,
     }
     print(json.dumps(line))
     return 0

@@ -194,6 +194,8 @@ def lib() -> ctypes.CDLL:
     L.chx_serve_get_stats.argtypes = [vp, ctypes.POINTER(ServeStats)]
     L.chx_serve_reset_stats.argtypes = [vp]
     L.chx_serve_latencies.argtypes = [vp, vp, vp, i64, pi64]
+    # test hook (not in the public header): the next launch fails on the host
+    L.chx_serve_debug_fail_next_launch.argtypes = [vp]
     _lib = L
     return L
 

@@ -14,7 +14,13 @@
 //   cut into time pieces that add atomically, so no workgroup idles).
 // Frames that arrive meanwhile are copied while the launch runs and form
 // the next launch.  Latency of a frame = submit -> completion of the launch
-// that integrated it, measured on the host clock.
+// that integrated it, measured on the host clock; the engine keeps the
+// records of the most recent kLatencyRecords frames.
+//
+// After a launch fails (a sticky error), the worker still retires every
+// queued frame without launching, so that reads and closes, which wait for
+// a stream's frames, return (with the error) and never free an accumulator
+// a queued frame still names.
 #include <string.h>
 
 #include <algorithm>
@@ -31,6 +37,9 @@
 namespace {
 
 using Clock = std::chrono::steady_clock;
+
+// Latency records kept (most recent frames; 16 MB at most).
+constexpr size_t kLatencyRecords = 1u << 20;
 
 struct Stream {
   int N, F, T, U;
@@ -85,6 +94,7 @@ struct chx_serve {
   std::thread worker;
   bool stop = false;
   int error = CHX_SUCCESS;  // sticky worker error
+  bool fail_next_launch = false;  // host-forced failure (chx_serve_debug_fail_next_launch)
 
   hipStream_t copy_s = nullptr, compute_s = nullptr, io_s = nullptr;
   hipEvent_t done_ev = nullptr;
@@ -105,7 +115,7 @@ struct chx_serve {
   size_t bytes_in_use = 0;
 
   // statistics
-  std::vector<std::pair<int64_t, double>> lat;  // (frame id, ms), completion order
+  std::deque<std::pair<int64_t, double>> lat;  // (frame id, ms), completion order, bounded
   int64_t frames_submitted = 0, frames_done = 0, batches = 0, batch_frames = 0;
   double cmacs = 0;
   bool have_first = false;
@@ -255,7 +265,15 @@ void worker_main(chx_serve *S) {
     while (!S->pending.empty() && (int)batch.size() < maxf) {
       const Frame &f = S->pending.front();
       if (f.copied && !batch.empty() && hipEventQuery(S->slot_ev[f.slot]) != hipSuccess) break;
-      const Stream &s = S->streams.at(f.stream);
+      auto sit = S->streams.find(f.stream);
+      if (sit == S->streams.end()) {  // cannot happen (closes drain first): retire it
+        S->open_frames.erase(f.id);
+        S->free_slots.push_back(f.slot);
+        S->pending.pop_front();
+        S->cv_done.notify_all();
+        continue;
+      }
+      const Stream &s = sit->second;
       const int64_t k = per_stream[f.stream];  // frames of this stream taken so far
       const int64_t grow = group_bytes(S->fmt, s, k + 1) - (k ? group_bytes(S->fmt, s, k) : 0);
       if (ws + grow > S->cfg.workspace_bytes) break;
@@ -269,7 +287,11 @@ void worker_main(chx_serve *S) {
     // Streams cannot close while they have frames in flight, so the
     // Stream pointers stay valid while unlocked.
     int rc = S->error;
-    if (rc == CHX_SUCCESS) {
+    if (rc == CHX_SUCCESS && S->fail_next_launch) {
+      S->fail_next_launch = false;
+      rc = CHX_ERR_HIP;
+    }
+    if (rc == CHX_SUCCESS && !batch.empty()) {
       lk.unlock();
       rc = run_launch(S, batch, st);
       lk.lock();
@@ -277,20 +299,23 @@ void worker_main(chx_serve *S) {
     const Clock::time_point now = Clock::now();
     if (rc != CHX_SUCCESS && S->error == CHX_SUCCESS) S->error = rc;
     for (const Frame &f : batch) {
-      Stream &s = S->streams.at(f.stream);
-      s.done++;
+      auto sit = S->streams.find(f.stream);
+      if (sit != S->streams.end()) sit->second.done++;
       S->open_frames.erase(f.id);
       S->free_slots.push_back(f.slot);
-      if (rc == CHX_SUCCESS) {
+      if (rc == CHX_SUCCESS && sit != S->streams.end()) {
         const double ms = std::chrono::duration<double, std::milli>(now - f.t_submit).count();
         S->lat.emplace_back(f.id, ms);
+        if (S->lat.size() > kLatencyRecords) S->lat.pop_front();
         S->frames_done++;
-        S->cmacs += s.cmacs_per_frame;
+        S->cmacs += sit->second.cmacs_per_frame;
       }
     }
-    S->batches++;
-    S->batch_frames += (int64_t)batch.size();
-    S->t_last = now;
+    if (!batch.empty()) {
+      S->batches++;
+      S->batch_frames += (int64_t)batch.size();
+      S->t_last = now;
+    }
     S->cv_done.notify_all();
   }
 }
@@ -313,13 +338,14 @@ void free_serve(chx_serve *S) {
   delete S;
 }
 
-// Waits (lock held) until every submitted frame of stream sid is done.
+// Waits (lock held) until every submitted frame of stream sid is done --
+// also after a sticky error, since the worker then retires queued frames
+// without launching them.
 int drain_stream(chx_serve *S, std::unique_lock<std::mutex> &lk, int64_t sid) {
   if (!S->streams.count(sid)) return CHX_ERR_NOT_FOUND;
   S->cv_done.wait(lk, [&] {
     auto it = S->streams.find(sid);
-    return it == S->streams.end() || it->second.done == it->second.submitted ||
-           S->error != CHX_SUCCESS;
+    return it == S->streams.end() || it->second.done == it->second.submitted;
   });
   if (!S->streams.count(sid)) return CHX_ERR_NOT_FOUND;  // closed meanwhile
   return S->error;
@@ -364,7 +390,8 @@ int submit(chx_serve *S, int64_t sid, const uint8_t *ptr, bool host, int64_t *fr
     lk.lock();
     if (e != hipSuccess) {  // undo the reservation
       const int rc = hip_status(e);
-      S->streams.at(sid).submitted--;
+      auto sit = S->streams.find(sid);
+      if (sit != S->streams.end()) sit->second.submitted--;
       S->frames_submitted--;
       S->open_frames.erase(f.id);
       S->free_slots.push_back(slot);
@@ -523,7 +550,9 @@ static int read_acc(chx_serve *S, int64_t sid, int32_t *dst, hipMemcpyKind kind)
   std::unique_lock<std::mutex> lk(S->mu);
   int rc = drain_stream(S, lk, sid);
   if (rc) return rc;
-  const Stream &s = S->streams.at(sid);
+  auto sit = S->streams.find(sid);
+  if (sit == S->streams.end()) return CHX_ERR_NOT_FOUND;
+  const Stream &s = sit->second;
   // The lock is held: no frame of this stream can be submitted meanwhile.
   CHX_HIP(hipSetDevice(S->cfg.device));
   CHX_HIP(hipMemcpyAsync(dst, s.acc, s.acc_bytes, kind, S->io_s));
@@ -544,7 +573,9 @@ int chx_serve_close(chx_serve *S, int64_t sid) {
   std::unique_lock<std::mutex> lk(S->mu);
   int rc = drain_stream(S, lk, sid);
   if (rc == CHX_ERR_NOT_FOUND) return rc;
-  Stream &s = S->streams.at(sid);
+  auto sit = S->streams.find(sid);
+  if (sit == S->streams.end()) return CHX_ERR_NOT_FOUND;
+  Stream &s = sit->second;
   (void)hipSetDevice(S->cfg.device);
   if (s.acc) (void)hipFree(s.acc);
   S->bytes_in_use -= s.acc_bytes;
@@ -554,25 +585,29 @@ int chx_serve_close(chx_serve *S, int64_t sid) {
 
 int chx_serve_get_stats(chx_serve *S, chx_serve_stats *o) {
   if (!S || !o) return CHX_ERR_INVALID_ARGUMENT;
-  std::lock_guard<std::mutex> g(S->mu);
+  std::vector<double> v;
+  int err;
   memset(o, 0, sizeof(*o));
-  o->frames_submitted = S->frames_submitted;
-  o->frames_done = S->frames_done;
-  o->batches = S->batches;
-  o->streams_open = (int64_t)S->streams.size();
-  o->bytes_in_use = (int64_t)S->bytes_in_use;
-  o->cmacs = S->cmacs;
-  o->mean_batch_frames = S->batches ? (double)S->batch_frames / S->batches : 0.0;
-  if (S->have_first && S->frames_done)
-    o->wall_seconds = std::chrono::duration<double>(S->t_last - S->t_first).count();
-  if (!S->lat.empty()) {
-    std::vector<double> v;
+  {
+    // copy under the lock; sort outside it, so the worker and submitters
+    // are not held up
+    std::lock_guard<std::mutex> g(S->mu);
+    o->frames_submitted = S->frames_submitted;
+    o->frames_done = S->frames_done;
+    o->batches = S->batches;
+    o->streams_open = (int64_t)S->streams.size();
+    o->bytes_in_use = (int64_t)S->bytes_in_use;
+    o->cmacs = S->cmacs;
+    o->mean_batch_frames = S->batches ? (double)S->batch_frames / S->batches : 0.0;
+    if (S->have_first && S->frames_done)
+      o->wall_seconds = std::chrono::duration<double>(S->t_last - S->t_first).count();
     v.reserve(S->lat.size());
+    for (auto &r : S->lat) v.push_back(r.second);
+    err = S->error;
+  }
+  if (!v.empty()) {
     double sum = 0;
-    for (auto &r : S->lat) {
-      v.push_back(r.second);
-      sum += r.second;
-    }
+    for (double x : v) sum += x;
     std::sort(v.begin(), v.end());
     auto pct = [&](double p) {  // nearest rank
       size_t k = (size_t)(p * (double)v.size() + 0.999999);
@@ -585,7 +620,7 @@ int chx_serve_get_stats(chx_serve *S, chx_serve_stats *o) {
     o->latency_max_ms = v.back();
     o->latency_mean_ms = sum / (double)v.size();
   }
-  return S->error;
+  return err;
 }
 
 int chx_serve_reset_stats(chx_serve *S) {
@@ -607,6 +642,15 @@ int chx_serve_latencies(chx_serve *S, int64_t *ids, double *ms, int64_t max, int
     if (ids) ids[k] = S->lat[k].first;
     if (ms) ms[k] = S->lat[k].second;
   }
+  return CHX_SUCCESS;
+}
+
+// Test hook, not in the public header: the next launch the worker forms
+// fails (sticky CHX_ERR_HIP) before anything is launched.
+int chx_serve_debug_fail_next_launch(chx_serve *S) {
+  if (!S) return CHX_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> g(S->mu);
+  S->fail_next_launch = true;
   return CHX_SUCCESS;
 }
 

@@ -309,17 +309,23 @@ int chx_serve_submit_device(chx_serve *serve, int64_t stream_id,
 /* Blocks until the frame's products are in its stream's accumulator. */
 int chx_serve_wait_frame(chx_serve *serve, int64_t frame_id);
 /* Blocks until every frame submitted to the stream is done, then copies its
- * accumulator ([F][NB][32][32][2] int32, block order) to host or device. */
+ * accumulator ([F][NB][32][32][2] int32, block order) to host or device.
+ * After a failed launch (a sticky error) the engine retires queued frames
+ * without integrating them; read and close still wait for them, then return
+ * the error (close also frees the stream). */
 int chx_serve_read(chx_serve *serve, int64_t stream_id, int32_t *h_vis);
 int chx_serve_read_device(chx_serve *serve, int64_t stream_id, int32_t *d_vis);
-/* Waits for the stream's frames, releases its accumulator. */
+/* Waits for the stream's frames (also after an error), releases its
+ * accumulator. */
 int chx_serve_close(chx_serve *serve, int64_t stream_id);
 int chx_serve_get_stats(chx_serve *serve, chx_serve_stats *stats);
 /* Clears the latency records and counters (not the streams). */
 int chx_serve_reset_stats(chx_serve *serve);
-/* Latency records of completed frames, oldest first: frame ids and
- * submit-to-done milliseconds.  Writes at most max records (either array
- * may be NULL) and the number of records held into *count. */
+/* Latency records of the most recent 2^20 completed frames (older ones are
+ * dropped; the percentiles of chx_serve_get_stats cover the same records),
+ * oldest first: frame ids and submit-to-done milliseconds.  Writes at most
+ * max records (either array may be NULL) and the number of records held
+ * into *count. */
 int chx_serve_latencies(chx_serve *serve, int64_t *frame_ids, double *ms, int64_t max,
                         int64_t *count);
 

@@ -9,6 +9,7 @@ length is checked, and the streaming workload (N=2048, F=8, 1024-sample
 frames, 5% of the elements held at (7,-8), 2^20 samples) integrates to the
 same visibilities as one Plan over the whole input.
 """
+import ctypes
 import sys
 import threading
 import time
@@ -205,3 +206,41 @@ def test_streaming_workload_2pow20_samples():
     NBn, nb = chx.num_blocks(N), N // 32
     b = hb * nb - hb * (hb - 1) // 2
     assert int(got[0, b, hl, hl, 0]) == 113 * n * T
+
+
+def test_close_and_read_after_a_failed_launch():
+    """After a launch fails (forced on the host: nothing reaches the GPU),
+    the engine's error is sticky; read and close of streams with frames
+    still queued wait until the worker has retired them, then return the
+    error -- and the accumulators they free are named by no queued frame."""
+    torch = _torch()
+    N, F, T = 256, 2, 1024
+    frames = [torch.randint(0, 256, (T, F, N), dtype=torch.uint8, device="cuda")
+              for _ in range(12)]
+    torch.cuda.synchronize()
+    with chx.Serve(frame_slots=12, slot_bytes=N * F * T, max_batch_frames=1,
+                   method=chx.FP6_MFMA) as sv:
+        a, b = sv.open(N, F, T, 1), sv.open(N, F, T, 0)
+        assert chx.lib().chx_serve_debug_fail_next_launch(sv._h) == 0
+        submitted = 0
+        for i, fr in enumerate(frames):
+            try:
+                sv.submit_device(a if i % 2 == 0 else b, fr.data_ptr())
+                submitted += 1
+            except chx.ChxError as e:  # the error is sticky once the worker hit it
+                assert e.status == chx.ERR_HIP
+                break
+        say(f"  submitted {submitted} frames before the error showed")
+        with pytest.raises(chx.ChxError) as e:
+            sv.close_stream(a)
+        assert e.value.status == chx.ERR_HIP
+        with pytest.raises(chx.ChxError) as e:
+            sv.read(b)
+        assert e.value.status == chx.ERR_HIP
+        with pytest.raises(chx.ChxError) as e:
+            sv.close_stream(b)
+        assert e.value.status == chx.ERR_HIP
+        st = chx.ServeStats()
+        assert chx.lib().chx_serve_get_stats(sv._h, ctypes.byref(st)) == chx.ERR_HIP
+        assert st.frames_submitted == submitted and st.frames_done == 0
+        assert st.streams_open == 0
