@@ -371,6 +371,23 @@ int chx_plan_buffers(chx_plan *p, uint8_t **d_input, int32_t **d_vis) {
   return CHX_SUCCESS;
 }
 
+int chx_device_alloc(int32_t device, size_t bytes, void **ptr) {
+  if (!ptr) return CHX_ERR_INVALID_ARGUMENT;
+  *ptr = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return CHX_ERR_NO_DEVICE;
+  if (device < 0 || device >= ndev) return CHX_ERR_NO_DEVICE;
+  CHX_HIP(hipSetDevice(device));
+  CHX_HIP(hipMalloc(ptr, bytes));
+  return CHX_SUCCESS;
+}
+
+int chx_device_free(void *ptr) {
+  if (!ptr) return CHX_ERR_INVALID_ARGUMENT;
+  CHX_HIP(hipFree(ptr));
+  return CHX_SUCCESS;
+}
+
 int chx_host_alloc(void **ptr, size_t bytes) {
   if (!ptr) return CHX_ERR_INVALID_ARGUMENT;
   CHX_HIP(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
@@ -397,6 +414,52 @@ int chx_copy_to_host(void *h_dst, const void *d_src, size_t bytes, void *stream)
 
 int chx_stream_synchronize(void *stream) {
   CHX_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return CHX_SUCCESS;
+}
+
+int chx_stream_create(int32_t device, void **stream) {
+  if (!stream) return CHX_ERR_INVALID_ARGUMENT;
+  *stream = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return CHX_ERR_NO_DEVICE;
+  if (device < 0 || device >= ndev) return CHX_ERR_NO_DEVICE;
+  CHX_HIP(hipSetDevice(device));
+  hipStream_t s = nullptr;
+  CHX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = (void *)s;
+  return CHX_SUCCESS;
+}
+
+int chx_stream_destroy(void *stream) {
+  if (!stream) return CHX_ERR_INVALID_ARGUMENT;
+  CHX_HIP(hipStreamDestroy((hipStream_t)stream));
+  return CHX_SUCCESS;
+}
+
+int chx_event_create(void **event) {
+  if (!event) return CHX_ERR_INVALID_ARGUMENT;
+  *event = nullptr;
+  hipEvent_t e = nullptr;
+  CHX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  *event = (void *)e;
+  return CHX_SUCCESS;
+}
+
+int chx_event_destroy(void *event) {
+  if (!event) return CHX_ERR_INVALID_ARGUMENT;
+  CHX_HIP(hipEventDestroy((hipEvent_t)event));
+  return CHX_SUCCESS;
+}
+
+int chx_event_record(void *event, void *stream) {
+  if (!event) return CHX_ERR_INVALID_ARGUMENT;
+  CHX_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+  return CHX_SUCCESS;
+}
+
+int chx_stream_wait_event(void *stream, void *event) {
+  if (!event) return CHX_ERR_INVALID_ARGUMENT;
+  CHX_HIP(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0));
   return CHX_SUCCESS;
 }
 

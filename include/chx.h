@@ -154,6 +154,10 @@ int chx_plan_kernel_times(chx_plan *plan, double *unpack_ms, double *gemm_ms,
 
 /* Allocates (once) and returns the plan's device input and output buffers. */
 int chx_plan_buffers(chx_plan *plan, uint8_t **d_input, int32_t **d_vis);
+/* Device memory on a device (clCreateBuffer, main_wrapper.c:455-501), e.g.
+ * a second stage of input and output buffers. */
+int chx_device_alloc(int32_t device, size_t bytes, void **ptr);
+int chx_device_free(void *ptr);
 /* Pinned host memory (hipHostMalloc), for fast H2D of large inputs. */
 int chx_host_alloc(void **ptr, size_t bytes);
 int chx_host_free(void *ptr);
@@ -162,6 +166,18 @@ int chx_copy_to_device(void *d_dst, const void *h_src, size_t bytes, void *strea
 int chx_copy_to_host(void *h_dst, const void *d_src, size_t bytes, void *stream);
 /* Waits for all work queued on stream (NULL = default stream). */
 int chx_stream_synchronize(void *stream);
+/* Queues and events for hosts that overlap copies with computation: replace
+ * the two out-of-order command queues and the event chaining of the
+ * reference loop (main_wrapper.c:264-274, 614-785).  A stream is a
+ * non-blocking hipStream_t on device; an event orders work across streams
+ * (chx_stream_wait_event: later work on stream waits for the work recorded
+ * before the event). */
+int chx_stream_create(int32_t device, void **stream);
+int chx_stream_destroy(void *stream);
+int chx_event_create(void **event);
+int chx_event_destroy(void *event);
+int chx_event_record(void *event, void *stream);
+int chx_stream_wait_event(void *stream, void *event);
 
 /* ---- Batch: many correlations of uneven shape in one graph launch. ----
  * No reference counterpart: the reference builds one program per shape and
