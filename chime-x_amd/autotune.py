@@ -2,12 +2,13 @@
 
 For a job list it times every candidate shape -- the matrix-core method
 (FP6 or int8 operands; bit-identical results), the work-item time extent
-(max_item_stages: whole chunks, or pieces of 64..512 stages whose partial
-sums add atomically), the persistent workgroup count, and the workspace
+(max_item_stages: whole chunks, or pieces of 128..1024 stages whose partial
+sums add atomically), the persistent workgroup count (one per compute unit,
+or fewer, which lowers the power the matrix cores draw), and the workspace
 budget (which sets the number of rounds) -- on the GPU with the captured
 graph, and keeps the fastest.  Results are cached in a JSON file keyed by
-the job list's shapes, so a later batch of the same shapes loads its tuning
-without timing anything.
+the GPU target and the job list's shapes, so a later batch of the same
+shapes on the same target loads its tuning without timing anything.
 
     python chime-x_amd/autotune.py --out chime-x_amd/tuning.json
 """
@@ -31,13 +32,13 @@ DEFAULT_CACHE = chx.TUNING_CACHE
 
 
 def signature(jobs) -> str:
-    return chx.job_signature(jobs)
+    return chx.job_signature(jobs, chx.device_arch(0))
 
 
 def candidates(cus: int):
     methods = (3, 2)  # FP6, int8 matrix cores (bit-identical results)
     stages = (0, 1024, 512, 256, 128)  # whole chunk (FP6: <= 2048 stages), or pieces
-    wgs = (cus,)
+    wgs = (cus, cus * 7 // 8 // 8 * 8, cus // 2 // 8 * 8)
     budgets = (16 << 30, 64 << 30)
     for m, st, wg, ws in itertools.product(methods, stages, wgs, budgets):
         yield dict(method=m, max_item_stages=st, workgroups=wg, workspace_bytes=ws)

@@ -153,6 +153,7 @@ def lib() -> ctypes.CDLL:
         getattr(L, name).argtypes = [vp, vp, vp, vp]
     L.chx_correlate_host.argtypes = [vp, vp, vp]
     L.chx_correlate_noalloc.argtypes = [ctypes.POINTER(Config), vp, vp, vp]
+    L.chx_device_arch.argtypes = [i32, ctypes.c_char_p, ctypes.c_size_t]
     L.chx_plan_profile.argtypes = [vp, i32]
     L.chx_plan_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double),
@@ -407,19 +408,31 @@ def batch_plan(jobs, tuning: dict | None = None) -> dict:
 TUNING_CACHE = os.path.join(HERE, "tuning.json")
 
 
-def job_signature(jobs) -> str:
-    """Key of a job list's shapes in the autotuner's cache."""
+def device_arch(device: int = 0) -> str | None:
+    """The GPU's target name ("gfx950"), or None without a GPU."""
+    buf = ctypes.create_string_buffer(64)
+    if lib().chx_device_arch(device, buf, len(buf)) != SUCCESS:
+        return None
+    return buf.value.decode()
+
+
+def job_signature(jobs, arch: str | None = None) -> str:
+    """Key of a job list's shapes on a GPU target in the autotuner's cache:
+    "<arch>/<hash of the sorted shapes>" (arch: the device's, "gfx950" when
+    none is visible)."""
     shapes = sorted((j["num_elements"], j["num_freq"], j["time_steps"],
                      j.get("upper_triangle_convention", 1)) for j in jobs)
-    return hashlib.sha1(json.dumps(shapes).encode()).hexdigest()[:16]
+    arch = arch or device_arch(0) or "gfx950"
+    return f"{arch}/" + hashlib.sha1(json.dumps(shapes).encode()).hexdigest()[:16]
 
 
-def cached_tuning(jobs, path: str = TUNING_CACHE):
-    """The autotuned launch shape of this job list's shapes, or None."""
+def cached_tuning(jobs, path: str = TUNING_CACHE, arch: str | None = None):
+    """The autotuned launch shape of this job list's shapes on this GPU
+    target, or None."""
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        ent = json.load(f).get(job_signature(jobs))
+        ent = json.load(f).get(job_signature(jobs, arch))
     return ent["best"]["tuning"] if ent else None
 
 

@@ -85,6 +85,23 @@ int32_t chx_max_time_accum(int32_t kernel_batch) {
   }
 }
 
+int chx_device_arch(int32_t device, char *name, size_t len) {
+  if (!name || len == 0) return CHX_ERR_INVALID_ARGUMENT;
+  name[0] = 0;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return CHX_ERR_NO_DEVICE;
+  if (device < 0 || device >= ndev) return CHX_ERR_NO_DEVICE;
+  hipDeviceProp_t prop;
+  CHX_HIP(hipGetDeviceProperties(&prop, device));
+  size_t k = 0;  // the target name, without its feature suffixes (":sramecc+:xnack-")
+  while (k + 1 < len && prop.gcnArchName[k] && prop.gcnArchName[k] != ':') {
+    name[k] = prop.gcnArchName[k];
+    k++;
+  }
+  name[k] = 0;
+  return CHX_SUCCESS;
+}
+
 int chx_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;

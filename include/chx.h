@@ -420,6 +420,10 @@ const char *chx_last_error_detail(void);
 /* Largest time_accum a method accepts (0 = no cap), -1 on a bad method. */
 int32_t chx_max_time_accum(int32_t kernel_batch);
 
+/* The GPU's target name (e.g. "gfx950") into name[len]; keys the
+ * autotuner's cache (no reference counterpart). */
+int chx_device_arch(int32_t device, char *name, size_t len);
+
 /* Number of visible GPUs (0 on a machine without one). */
 int chx_device_count(void);
 
