@@ -86,7 +86,9 @@ extern "C" int chx_correlate_noalloc(const chx_config *cfg, const uint8_t *d_inp
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return CHX_ERR_NO_DEVICE;
   if (cfg->device < 0 || cfg->device >= ndev || cfg->device >= kMaxDevices)
     return CHX_ERR_NO_DEVICE;
-  CHX_HIP(hipSetDevice(cfg->device));
+  int cur = -1;
+  CHX_HIP(hipGetDevice(&cur));
+  if (cur != cfg->device) CHX_HIP(hipSetDevice(cfg->device));
   uint32_t *base = nullptr;
   rc = scratch_ptr(cfg->device, &base);
   if (rc) return rc;

@@ -402,3 +402,36 @@ def test_config2_full_size_all_methods():
     got = ref[:, torch.as_tensor(b, device="cuda"), torch.as_tensor(Y % 32, device="cuda"),
               torch.as_tensor(X % 32, device="cuda")].cpu().numpy()
     np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_plan_correlate_in_a_captured_graph(method):
+    # chx_correlate captured into a graph (fills are kernels, not memset
+    # nodes), replayed several times on new input with eager work and host
+    # synchronisation between the replays: every replay equals the eager
+    # result and the oracle.
+    torch = _torch()
+    T, F, N = 2048, 2, 256
+    rng = np.random.default_rng(31)
+    with chx.Plan(N, F, T, 256, 1, method) as p:
+        d_in = torch.empty((T, F, N), dtype=torch.uint8, device="cuda")
+        out = torch.empty(p.output_shape, dtype=torch.int32, device="cuda")
+        eager = torch.empty_like(out)
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                p.correlate_device(d_in.data_ptr(), out.data_ptr(), s.cuda_stream)
+        torch.cuda.synchronize()
+        for rep in range(3):
+            data = rng.integers(0, 256, size=(T, F, N), dtype=np.uint8)
+            d_in.copy_(torch.from_numpy(data))
+            p.correlate_device(d_in.data_ptr(), eager.data_ptr())
+            torch.cuda.synchronize()
+            out.fill_(-1)
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(out, eager), f"replay {rep}"
+            np.testing.assert_array_equal(out.cpu().numpy(), O.correlate_blocks(data, 1))

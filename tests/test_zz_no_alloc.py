@@ -76,3 +76,38 @@ def test_noalloc_leaves_free_memory_unchanged():
     torch.cuda.synchronize()
     assert torch.cuda.mem_get_info()[0] == free0
     np.testing.assert_array_equal(out.cpu().numpy(), O.correlate_blocks(data, 1))
+
+
+def test_noalloc_in_a_captured_graph_replays():
+    """Captured into a graph (its scratch is rewritten by kernels, no memset
+    nodes), replayed several times with eager work and host synchronisation
+    between the replays, each replay on new input equals chx_correlate."""
+    torch = _torch()
+    T, F, N, ta = 1024, 2, 192, 256
+    rng = np.random.default_rng(77)
+    d_in = torch.empty((T, F, N), dtype=torch.uint8, device="cuda")
+    out = torch.empty((F, chx.num_blocks(N), 32, 32, 2), dtype=torch.int32, device="cuda")
+    chx.correlate_noalloc(N, F, T, d_in.data_ptr(), out.data_ptr(), ta, 1, chx.HIGHLY_PACKED)
+    torch.cuda.synchronize()  # resolves the scratch address before capture
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            chx.correlate_noalloc(N, F, T, d_in.data_ptr(), out.data_ptr(), ta, 1,
+                                  chx.HIGHLY_PACKED, stream=s.cuda_stream)
+    torch.cuda.synchronize()
+    for rep in range(4):
+        data = rng.integers(0, 256, size=(T, F, N), dtype=np.uint8)
+        d_in.copy_(torch.from_numpy(data))
+        # eager work of the program between replays (another shape on the
+        # same scratch, then torch's own kernels)
+        other = torch.empty((1, chx.num_blocks(64), 32, 32, 2), dtype=torch.int32,
+                            device="cuda")
+        x64 = torch.randint(0, 256, (256, 1, 64), dtype=torch.uint8, device="cuda")
+        chx.correlate_noalloc(64, 1, 256, x64.data_ptr(), other.data_ptr(), 64)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        want = _plan_output(torch, d_in, T, F, N, ta, 1, chx.HIGHLY_PACKED)
+        assert torch.equal(out, want), f"replay {rep}"
