@@ -3,7 +3,9 @@
 For a job list it times every candidate shape -- the matrix-core method
 (FP6 or int8 operands; bit-identical results), the work-item time extent
 (max_item_stages: whole chunks, or pieces of 128..1024 stages whose partial
-sums add atomically), the persistent workgroup count (one per compute unit,
+sums add atomically), the output tile shape (FP6: 128 x 128 tiles for arrays
+of up to 256 or 512 elements, 128 x 256 otherwise), the persistent
+workgroup count (one per compute unit,
 or fewer, which lowers the power the matrix cores draw), and the workspace
 budget (which sets the number of rounds) -- on the GPU with the captured
 graph, and keeps the fastest.  Results are cached in a JSON file keyed by
@@ -41,7 +43,11 @@ def candidates(cus: int):
     wgs = (cus, cus * 7 // 8 // 8 * 8, cus // 2 // 8 * 8)
     budgets = (16 << 30, 64 << 30)
     for m, st, wg, ws in itertools.product(methods, stages, wgs, budgets):
-        yield dict(method=m, max_item_stages=st, workgroups=wg, workspace_bytes=ws)
+        # tile shape (FP6): 128 x 128 tiles for arrays up to this many
+        # elements, 128 x 256 above
+        for small in ((0, 256, 512) if m == 3 else (0,)):
+            yield dict(method=m, max_item_stages=st, workgroups=wg, workspace_bytes=ws,
+                       small_tile_elements=small)
 
 
 def time_batch(descs, tuning, reps: int = 3) -> float:
@@ -100,13 +106,16 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--out", default=DEFAULT_CACHE)
+    ap.add_argument("--sizes", default="256,512,1024,2048",
+                    help="array sizes the job list draws from")
     ap.add_argument("--replay", type=int, default=0,
                     help="no tuning: replay the cached shape this many times and report it")
     a = ap.parse_args()
     if not torch.cuda.is_available():
         print("autotune needs a GPU", file=sys.stderr)
         return 2
-    jobs = W.uneven_batch_jobs(a.njobs, a.seed)
+    jobs = W.uneven_batch_jobs(a.njobs, a.seed,
+                               sizes=tuple(int(x) for x in a.sizes.split(",")))
     ins, outs, descs = [], [], []
     for j in jobs:
         N, F, T = j["num_elements"], j["num_freq"], j["time_steps"]

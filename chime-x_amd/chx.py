@@ -80,7 +80,7 @@ class BatchTuning(ctypes.Structure):
         ("workgroups", ctypes.c_int32),
         ("workspace_bytes", ctypes.c_int64),
         ("method", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("small_tile_elements", ctypes.c_int32),
     ]
 
 
@@ -95,7 +95,7 @@ class BatchInfo(ctypes.Structure):
         ("workspace_bytes", ctypes.c_size_t),
         ("cmacs", ctypes.c_int64),
         ("method", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("small_tile_jobs", ctypes.c_int32),
     ]
 
 
@@ -391,7 +391,8 @@ def _tuning(tuning):
     if not tuning:
         return None
     return BatchTuning(tuning.get("max_item_stages", 0), tuning.get("workgroups", 0),
-                       tuning.get("workspace_bytes", 0), tuning.get("method", 0), 0)
+                       tuning.get("workspace_bytes", 0), tuning.get("method", 0),
+                       tuning.get("small_tile_elements", 0))
 
 
 def batch_plan(jobs, tuning: dict | None = None) -> dict:

@@ -7,8 +7,9 @@
 //    job, sized so that one round's planes of all jobs still active fit the
 //    workspace budget; a job takes part in the rounds its length reaches;
 //  * per round: one batched unpack launch over every active job's chunk, then
-//    one persistent GEMM launch over work items (job, channel, 128x256 tile,
-//    stage range); a job's tiles are cut in time when max_item_stages asks
+//    one persistent GEMM launch over work items (job, channel, 128x256 tile
+//    -- or, FP6, 128x128 for arrays up to small_tile_elements -- and stage
+//    range); a job's tiles are cut in time when max_item_stages asks
 //    or the chunk exceeds the item cap (kMaxItemStages), in which case their
 //    partial sums add atomically;
 //  * items are grouped per (job, channel) and the groups spread over 8
@@ -57,10 +58,10 @@ struct chx_batch {
 namespace chx {
 
 int add_job_items(int fmt, int gj, int F, int ntiles, int nst, int max_st,
-                  std::vector<ItemGroup> &groups) {
+                  std::vector<ItemGroup> &groups, bool small) {
   const int per = format_item_stages(fmt, nst, max_st);
   const int pieces = (nst + per - 1) / per;
-  const int tiles = (int)format_tiles(fmt, ntiles);
+  const int tiles = (int)(small ? small_tiles(ntiles) : format_tiles(fmt, ntiles));
   for (int f = 0; f < F; f++) {
     ItemGroup G;
     G.work = 0;
@@ -69,7 +70,8 @@ int add_job_items(int fmt, int gj, int F, int ntiles, int nst, int max_st,
       for (int k = 0; k < pieces; k++) {
         const int s0 = k * per, ns = per < nst - s0 ? per : nst - s0;
         G.items.push_back({gj, f, t, s0, ns});
-        G.work += ns + 8;  // + a flush/prologue allowance
+        // + a flush/prologue allowance; a small tile does half the MFMAs
+        G.work += (small ? ns / 2 : ns) + 8;
       }
     groups.push_back(std::move(G));
   }
@@ -179,12 +181,13 @@ int check_launch_tables(int fmt, const std::vector<UnpackJob> &ujobs,
         format_kb(fmt, stages) != g.KB || g.ntiles * kMfmaTile != g.Np || region > 0x7FFFFFFFLL)
       return CHX_ERR_HIP;
     if (fmt == kPlanesFp6 && g.planes1 != g.planes + 64LL * g.F * g.KB * g.Np) return CHX_ERR_HIP;
+    if (g.small && fmt != kPlanesFp6) return CHX_ERR_HIP;
   }
   for (const GemmItem &w : items) {
     if (w.job < 0 || w.job >= (int)gjobs.size()) return CHX_ERR_HIP;
     const GemmJob &g = gjobs[w.job];
     const int stages = fmt == kPlanesFp6 ? g.KB : g.KB / 4;
-    if (w.f < 0 || w.f >= g.F || w.tile < 0 || w.tile >= format_tiles(fmt, g.ntiles) ||
+    if (w.f < 0 || w.f >= g.F || w.tile < 0 || w.tile >= job_tiles(fmt, g) ||
         w.stage0 < 0 || w.nstages < ring || w.nstages % ring || w.stage0 + w.nstages > stages ||
         w.nstages > format_max_stages(fmt))
       return CHX_ERR_HIP;
@@ -346,9 +349,12 @@ int plan_tables(chx_batch *b, uint8_t *base, const std::vector<int64_t> &off, Ta
       ujobs.push_back(u);
       R.units += unpack_units(b->fmt, u);
 
-      // Items of this job: whole chunk per tile, or max_st-stage pieces.
+      // Items of this job: whole chunk per tile, or max_st-stage pieces;
+      // 128 x 128 tiles for arrays of at most small_tile_elements (FP6).
+      const bool small = b->fmt == kPlanesFp6 && N <= b->tune.small_tile_elements;
       const int per = format_item_stages(b->fmt, nst, max_st);
-      const int pieces = add_job_items(b->fmt, (int)gjobs.size(), F, nt, nst, max_st, groups);
+      const int pieces =
+          add_job_items(b->fmt, (int)gjobs.size(), F, nt, nst, max_st, groups, small);
       GemmJob g;
       g.planes = planes;
       g.planes1 = planes1;
@@ -362,6 +368,7 @@ int plan_tables(chx_batch *b, uint8_t *base, const std::vector<int64_t> &off, Ta
       g.U = d.upper_triangle_convention;
       g.mode = pieces > 1 ? 2 : (r == 0 ? 0 : 1);
       g.stages_per_split = per;
+      g.small = small ? 1 : 0;
       if (pieces > 1 && r == 0) b->zero.push_back({d.d_vis, (int64_t)F * g.NB * 2048});
       gjobs.push_back(g);
     }
@@ -423,12 +430,13 @@ int prepare(const chx_job_desc *jobs, int32_t num_jobs, const chx_batch_tuning *
   }
   t = tuning ? *tuning : chx_batch_tuning{};
   if (t.max_item_stages < 0 || t.workgroups < 0 || t.workgroups % 8 != 0 ||
-      t.workspace_bytes < 0)
+      t.workspace_bytes < 0 || t.small_tile_elements < 0)
     return CHX_ERR_INVALID_ARGUMENT;
   if (t.method == 0) t.method = CHX_METHOD_FP6_MFMA;
   if (t.method != CHX_METHOD_INT8_MFMA && t.method != CHX_METHOD_FP6_MFMA)
     return CHX_ERR_INVALID_ARGUMENT;
   fmt = t.method == CHX_METHOD_FP6_MFMA ? kPlanesFp6 : kPlanesInt8;
+  if (fmt != kPlanesFp6) t.small_tile_elements = 0;  // 128 x 128 tiles: FP6 only
   if (t.workspace_bytes == 0) t.workspace_bytes = kDefaultBudget;
   t.max_item_stages = (int32_t)round_up(t.max_item_stages, format_ring(fmt));
   // The smallest round (one ring of every job) must fit the budget.
@@ -541,6 +549,9 @@ int chx_batch_get_info(const chx_batch *b, chx_batch_info *info) {
   info->workgroups = b->tune.workgroups;
   info->max_item_stages = b->tune.max_item_stages;
   info->method = b->tune.method;
+  for (const chx_job_desc &j : b->jobs)
+    if (b->fmt == kPlanesFp6 && j.num_elements <= b->tune.small_tile_elements)
+      info->small_tile_jobs++;
   info->workspace_bytes = b->work_bytes;
   info->cmacs = b->cmacs;
   return CHX_SUCCESS;

@@ -298,6 +298,18 @@ __device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti_a, int &
   }
 }
 
+// Small tile tb of one channel: row-major over the upper triangle of
+// 128 x 128 tiles (row tile ti holds ntiles - ti of them).
+__device__ __forceinline__ void tile_coords_small(int tb, int ntiles, int &ti, int &ct) {
+  int r = 0;
+  while (r < ntiles - 1 && tb >= ntiles - r) {
+    tb -= ntiles - r;
+    r++;
+  }
+  ti = r;
+  ct = r + tb;
+}
+
 // Pass 2, one output item of channel f over k-steps [st_begin, st_begin +
 // nst) (nst a multiple of kRing).  Wave w computes 128 rows x 64 columns as
 // 4x2 blocks of two FP32 accumulators (256 AGPRs):
@@ -305,10 +317,14 @@ __device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti_a, int &
 //          tile; the four waves share the rows);
 //    PAIR: two half tiles of column tile ct: waves 0-1 rows from kTile
 //          ti_a, waves 2-3 rows from kTile ti_b, columns kTile ct + 64 (w%2).
+//   SMALL: a 128 x 128 tile (rows kTile ti_a, columns kTile ct): wave w
+//          computes rows 64 (w/2) .. + 63 x columns 64 (w%2) .. + 63 as
+//          2x2 blocks of two FP32 accumulators (the four waves share the
+//          rows' LDS ring).
 // Row operands (4 groups x 2 planes per 128-row block) go through the LDS
 // ring by buffer->LDS DMA; each wave loads its own column operands (2
 // groups x 2 planes) straight into VGPRs, three stages ahead.
-template <bool PAIR>
+template <bool PAIR, bool SMALL = false>
 __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, const int ti_a,
                                              const int ti_b, const int ct, const int st_begin,
                                              const int nst, uint8_t (*lds)[kSlotBytes]) {
@@ -339,6 +355,9 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   constexpr int kDma = PAIR ? 6 : 3;  // row DMAs per wave and stage
   constexpr int kP0n = PAIR ? 4 : 2, kP1n = PAIR ? 2 : 1;
   static_assert(kDma == kP0n + kP1n, "the waits below count kDma + 8 loads per stage");
+  static_assert(!(PAIR && SMALL), "a small tile is one 128-row block");
+  constexpr int kG = SMALL ? 2 : 4;                  // row groups this wave computes
+  const int gr0 = SMALL ? 2 * (wave >> 1) : 0;       // the first of them
   const int rb = PAIR ? wave >> 1 : 0;               // this wave's row block
   const int lp = PAIR ? wave & 1 : wave >> 1;        // plane it fills
   const int g0 = PAIR ? 0 : 2 * (wave & 1);          // first row group it fills
@@ -374,8 +393,8 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   // -> (half lane/32, element + lane%32).  Columns past the array (waves 2
   // and 3 of an unpaired half tile) load through an empty descriptor, which
   // fetches nothing and returns zeros; their products are never stored.
-  const bool past = !PAIR && wave >= 2 && (ct + 1) * kTile >= Np;
-  const int cw = PAIR ? wave & 1 : wave;  // column quarter of the tile
+  const bool past = !PAIR && !SMALL && wave >= 2 && (ct + 1) * kTile >= Np;
+  const int cw = PAIR || SMALL ? wave & 1 : wave;  // column quarter of the tile
   const __amdgpu_buffer_rsrc_t rz = __builtin_amdgcn_make_buffer_rsrc(
       (void *)J.planes, (short)0, 0, 0x00020000);
   __amdgpu_buffer_rsrc_t c0[2], c1[2];
@@ -408,10 +427,10 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
     I1 = load_op_g(1, 1, s0, s1);
   };
 
-  v16f re[4][2], im[4][2];
+  v16f re[kG][2], im[kG][2];
   const v16f zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < kG; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) re[i][j] = im[i][j] = zero;
   const uint8_t *lds0 = &lds[0][0];
@@ -438,7 +457,7 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
                     (int)0x82082082u, 0, 0};
   // Row operands: two sets, alternating by stage parity: a stage computes
   // with one and reads the next stage's into the other.
-  v8i AR[2][4], AI[2][4];
+  v8i AR[2][kG], AI[2][kG];
   // One stage with column operands (br0, br1, bi0, bi1) and row set p:
   // RE += Ar.Br + Ai.Bi, IM += Ar.Bi + Ai.(-Br), -Br being Br with the
   // sign bit of every code flipped.  The next stage's row operands (slot
@@ -448,7 +467,7 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
     const v8i br[2] = {br0, br1}, bi[2] = {bi0, bi1};
     const v8i nbr[2] = {br0 ^ sign, br1 ^ sign};
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < kG; i++) {
       const v8i xr = AR[p][i], xi = AI[p][i];
 #pragma unroll
       for (int j = 0; j < 2; j++) {
@@ -464,11 +483,11 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
         im[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
             xi, nbr[j], im[i][j], kFmtE2M3, kFmtE2M3, 0, kScale2p3, 0, kScale2p3);
       }
-      AR[p ^ 1][i] = load_op(nslot, 0, i);
-      AI[p ^ 1][i] = load_op(nslot, 1, i);
+      AR[p ^ 1][i] = load_op(nslot, 0, gr0 + i);
+      AI[p ^ 1][i] = load_op(nslot, 1, gr0 + i);
     }
   };
-  const int by0 = ti * kTile / 32, bx0 = (ct * kTile + 64 * cw) / 32;
+  const int by0 = ti * kTile / 32 + gr0, bx0 = (ct * kTile + 64 * cw) / 32;
 
   // Issue groups: G(st), issued during stage st, = the row DMAs of stage
   // st+4 (kDma) and the column loads of stage st+3 (8).  Prologue: G(-4) =
@@ -496,9 +515,9 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   else
     asm volatile("s_waitcnt vmcnt(33)\n\ts_barrier" ::: "memory");
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    AR[0][i] = load_op(0, 0, i);
-    AI[0][i] = load_op(0, 1, i);
+  for (int i = 0; i < kG; i++) {
+    AR[0][i] = load_op(0, 0, gr0 + i);
+    AI[0][i] = load_op(0, 1, gr0 + i);
   }
 
   // Stage st (slot st % kRing):
@@ -522,21 +541,24 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
       stage_ops(BR[d][0], BR[d][1], BI[d][0], BI[d][1], (d + 1) % kRing, d & 1);
       // 8 MFMAs per row group i, each group with 4 LDS reads of the next
       // stage's row operands; VMEM ops (kDma DMAs, 8 column loads) over
-      // groups 0-2
+      // groups 0-2 (SMALL: over its two groups)
 #pragma unroll
-      for (int g = 0; g < 4; g++)
+      for (int g = 0; g < kG; g++)
 #pragma unroll
         for (int i = 0; i < 8; i++) {
           __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
           if (i % 2 == 1) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-          if (g < 3 && i < (PAIR ? (g == 2 ? 4 : 5) : (g == 2 ? 3 : 4)))
+          if (SMALL) {
+            if (i < (g == 0 ? 6 : 5)) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+          } else if (g < 3 && i < (PAIR ? (g == 2 ? 4 : 5) : (g == 2 ? 3 : 4))) {
             __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+          }
         }
       __builtin_amdgcn_sched_barrier(0);
     }
   }
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < kG; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) {
       const int by = by0 + i, bx = bx0 + j;
@@ -613,6 +635,11 @@ __global__ __launch_bounds__(256, 1) void corr_f6_batch_kernel(
     const GemmItem w = items[it];
     const GemmJob J = jobs[w.job];
     int ti_a, ti_b, ct;
+    if (J.small) {
+      tile_coords_small(w.tile, J.ntiles, ti_a, ct);
+      gemm_tile_f6<false, true>(J, w.f, ti_a, ti_a, ct, w.stage0, w.nstages, lds);
+      continue;
+    }
     tile_coords(w.tile, J.ntiles, ti_a, ti_b, ct);
     if (ti_b >= 0 && ti_b != ti_a)
       gemm_tile_f6<true>(J, w.f, ti_a, ti_b, ct, w.stage0, w.nstages, lds);
@@ -738,6 +765,7 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
     J.U = U;
     J.mode = kmode;
     J.stages_per_split = sps;
+    J.small = 0;
     const int64_t grid = head + tail * split;
     if (grid > 0x7FFFFFFF) return CHX_ERR_SHAPE;
     corr_f6_kernel<<<dim3((unsigned)grid), dim3(256), 0, s>>>(J, (int)head, (int)tail, tmode);

@@ -102,6 +102,8 @@ inline int format_ring(int fmt) { return fmt == kPlanesFp6 ? kFp6Ring : kMfmaRin
 // work items of one channel (both formats: tiles of 128 x 256, half tiles
 // in pairs)
 inline int64_t format_tiles(int, int64_t ntiles) { return paired_tiles(ntiles); }
+// work items of one channel on 128 x 128 tiles: the upper triangle of tiles
+__host__ __device__ inline int64_t small_tiles(int64_t ntiles) { return ntiles * (ntiles + 1) / 2; }
 // the most stages one work item may cover
 inline int format_max_stages(int) { return kMaxItemStages; }
 // stages per work item of an nst-stage chunk cut at max_st (0 = whole)
@@ -123,7 +125,12 @@ struct GemmJob {
   int32_t *vis;            // block-order output
   int64_t NB;              // blocks per channel
   int F, Np, KB, ntiles, nb, U, mode, stages_per_split;  // mode 0 =, 1 +=, 2 atomic +=
+  int small;  // fp6 batch only: 128 x 128 tiles (four waves of 64 x 64) instead of 128 x 256
 };
+// work items of one channel of a GEMM job
+inline int64_t job_tiles(int fmt, const GemmJob &g) {
+  return g.small ? small_tiles(g.ntiles) : format_tiles(fmt, g.ntiles);
+}
 // One work item of the persistent batch kernel: tile `tile` (row-major over
 // the upper triangle of tiles) of channel f of jobs[job], over stages
 // [stage0, stage0 + nstages) of the chunk (nstages a multiple of the ring).
@@ -167,10 +174,10 @@ struct ItemGroup {
 };
 // Appends the groups of GEMM job gj (F channels, ntiles 128-element tiles
 // per side, nst stages cut into pieces of format_item_stages(fmt, nst,
-// max_st) stages): one group per channel.  Returns the number of pieces per
-// tile.
+// max_st) stages; small: 128 x 128 tiles): one group per channel.  Returns
+// the number of pieces per tile.
 int add_job_items(int fmt, int gj, int F, int ntiles, int nst, int max_st,
-                  std::vector<ItemGroup> &groups);
+                  std::vector<ItemGroup> &groups, bool small = false);
 // Longest-processing-time-first assignment of the groups to 8 queues (long
 // items first inside a queue); appends the items to `items` and the 9 queue
 // bounds to `qbound`.

@@ -617,6 +617,7 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
     J.U = U;
     J.mode = kmode;
     J.stages_per_split = sps;
+    J.small = 0;
     const int64_t grid = head + tail * split;
     if (grid > 0x7FFFFFFF) return CHX_ERR_SHAPE;
     corr_mfma_kernel<<<dim3((unsigned)grid), dim3(256), 0, s>>>(J, nst, (int)head, (int)tail,

@@ -11,14 +11,15 @@ import numpy as np
 UNEVEN_N = (256, 512, 1024, 2048)
 
 
-def uneven_batch_jobs(njobs: int = 64, seed: int = 3, U: int = 1) -> list[dict]:
-    """The uneven batch: N from {256, 512, 1024, 2048}, 1-8 channels, an
-    integration length in [4096, 131072] (a multiple of 256), and a data seed
-    per job."""
+def uneven_batch_jobs(njobs: int = 64, seed: int = 3, U: int = 1,
+                      sizes: tuple = UNEVEN_N) -> list[dict]:
+    """The uneven batch: N from {256, 512, 1024, 2048} (or `sizes`), 1-8
+    channels, an integration length in [4096, 131072] (a multiple of 256),
+    and a data seed per job."""
     rng = np.random.default_rng(seed)
     jobs = []
     for k in range(njobs):
-        jobs.append(dict(num_elements=int(rng.choice(UNEVEN_N)),
+        jobs.append(dict(num_elements=int(rng.choice(sizes)),
                          num_freq=int(rng.integers(1, 9)),
                          time_steps=int(rng.integers(4096 // 256, 131072 // 256 + 1)) * 256,
                          upper_triangle_convention=U,

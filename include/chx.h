@@ -212,7 +212,10 @@ typedef struct chx_batch_tuning {
                                0 = 16 GiB */
   int32_t method;           /* CHX_METHOD_INT8_MFMA or CHX_METHOD_FP6_MFMA;
                                0 = FP6 */
-  int32_t reserved;         /* 0 */
+  int32_t small_tile_elements; /* FP6 only: jobs of at most this many
+                               elements use output tiles of 128 x 128 (four
+                               waves of 64 x 64) instead of 128 x 256;
+                               0 = none (the int8 method ignores it) */
 } chx_batch_tuning;
 
 typedef struct chx_batch_info {
@@ -225,7 +228,7 @@ typedef struct chx_batch_info {
   size_t workspace_bytes;  /* allocated planes workspace */
   int64_t cmacs;           /* sum over jobs of F * T * N(N+1)/2 */
   int32_t method;          /* matrix-core method used */
-  int32_t reserved;
+  int32_t small_tile_jobs; /* jobs on 128 x 128 tiles */
 } chx_batch_info;
 
 typedef struct chx_batch chx_batch;

@@ -206,9 +206,22 @@ def _item_blocks(nt):
     return items
 
 
-def _expected_items(jobs, chunk, max_st, unit, method):
+def _small_item_blocks(nt):
+    """The 32x32 blocks each 128 x 128 work item stores: tiles row-major over
+    the upper triangle of 128-element tiles, wave w computing rows
+    64 (w/2).. and columns 64 (w%2).. of its tile as 2x2 blocks."""
+    items = []
+    for ti in range(nt):
+        for ct in range(ti, nt):
+            items.append({(4 * ti + 2 * (w >> 1) + i, 4 * ct + 2 * (w & 1) + j)
+                          for w in range(4) for i in range(2) for j in range(2)})
+    return items
+
+
+def _expected_items(jobs, chunk, max_st, unit, method, small=0):
     """Work items of the plan: per round, per active job, F x tiles x pieces
-    (pieces hold at most 2048 stages)."""
+    (pieces hold at most 2048 stages; FP6 jobs of at most `small` elements
+    on 128 x 128 tiles)."""
     n, rounds = 0, -(-max(j["time_steps"] for j in jobs) // chunk)
     for r in range(rounds):
         for j in jobs:
@@ -219,7 +232,9 @@ def _expected_items(jobs, chunk, max_st, unit, method):
             per = max_st if 0 < max_st < nst else nst
             per = min(per, 2048)
             nt = -(-j["num_elements"] // 128)
-            n += j["num_freq"] * _tiles(nt, method) * -(-nst // per)
+            tiles = (nt * (nt + 1) // 2 if method == 3 and j["num_elements"] <= small
+                     else _tiles(nt, method))
+            n += j["num_freq"] * tiles * -(-nst // per)
     return rounds, n
 
 
@@ -249,10 +264,28 @@ def test_items_cover_the_upper_triangle_once():
             assert all(bx < 4 * nt for (_, bx) in blocks)
 
 
+def test_small_tiles_cover_the_upper_triangle_once():
+    # With 128 x 128 tiles every upper-triangle block is stored by exactly
+    # one work item, and nothing is computed past the array.
+    for nt in range(1, 24):
+        items = _small_item_blocks(nt)
+        assert len(items) == nt * (nt + 1) // 2
+        seen = {}
+        for k, blocks in enumerate(items):
+            assert all(bx < 4 * nt and by < 4 * nt for (by, bx) in blocks)
+            for (by, bx) in blocks:
+                if bx >= by:
+                    assert (by, bx) not in seen, (nt, by, bx)
+                    seen[(by, bx)] = k
+        assert len(seen) == 4 * nt * (4 * nt + 1) // 2
+
+
 @pytest.mark.parametrize("method", [2, 3])
 @pytest.mark.parametrize("tuning", [None, dict(max_item_stages=64), dict(max_item_stages=6),
                                     dict(workspace_bytes=1 << 30),
-                                    dict(workspace_bytes=3 << 30, max_item_stages=512)])
+                                    dict(workspace_bytes=3 << 30, max_item_stages=512),
+                                    dict(small_tile_elements=512),
+                                    dict(small_tile_elements=4096, max_item_stages=256)])
 def test_batch_plan_host_only(tuning, method):
     import workloads as W
     jobs = W.uneven_batch_jobs()
@@ -267,7 +300,8 @@ def test_batch_plan_host_only(tuning, method):
     maxT = max(j["time_steps"] for j in jobs)
     chunk = min(budget // per_unit * unit, -(-maxT // unit) * unit)
     max_st = -(-t.get("max_item_stages", 0) // ring) * ring
-    rounds, items = _expected_items(jobs, chunk, max_st, unit, method)
+    small = t.get("small_tile_elements", 0)
+    rounds, items = _expected_items(jobs, chunk, max_st, unit, method, small)
     assert info["chunk_samples"] == chunk
     assert info["rounds"] == rounds and info["items"] == items
     assert info["max_item_stages"] == max_st
@@ -276,6 +310,8 @@ def test_batch_plan_host_only(tuning, method):
                                 for j in jobs)
     assert info["num_jobs"] == 64
     assert info["method"] == method
+    assert info["small_tile_jobs"] == (sum(j["num_elements"] <= small for j in jobs)
+                                       if method == 3 else 0)
 
 
 @pytest.mark.parametrize("method", [2, 3])

@@ -70,6 +70,8 @@ def mixed_oracle():
     dict(max_item_stages=8, workgroups=16),
     dict(workspace_bytes=MIXED_PER_SAMPLE * 1024),
     dict(workspace_bytes=MIXED_PER_SAMPLE * 512, max_item_stages=4, workgroups=8),
+    dict(small_tile_elements=4096),
+    dict(small_tile_elements=160, max_item_stages=4, workgroups=16),
 ])
 @pytest.mark.parametrize("eager", [True, False])
 @pytest.mark.parametrize("method", [chx.INT8_MFMA, chx.FP6_MFMA])
@@ -156,11 +158,13 @@ def _pairs_check(torch, d_in, d_vis, N, U, rng, nsub=16):
     np.testing.assert_array_equal(got, want)
 
 
-@pytest.mark.parametrize("method", [chx.INT8_MFMA, chx.FP6_MFMA])
-def test_uneven_batch_64_jobs_matches_alone_and_oracle(method):
+@pytest.mark.parametrize("method,small", [(chx.INT8_MFMA, 0), (chx.FP6_MFMA, 0),
+                                          (chx.FP6_MFMA, 512)])
+def test_uneven_batch_64_jobs_matches_alone_and_oracle(method, small):
     """The 64-job uneven batch: each job's output from the one graph launch
     equals the job correlated alone (its own Plan) bit for bit, and the
-    oracle on a 16-element subset of every job."""
+    oracle on a 16-element subset of every job; FP6 also with the arrays of
+    up to 512 elements on 128 x 128 tiles."""
     torch = _torch()
     jobs = W.uneven_batch_jobs()
     ins, outs, descs = [], [], []
@@ -171,7 +175,7 @@ def test_uneven_batch_64_jobs_matches_alone_and_oracle(method):
                                 device="cuda"))
         descs.append(dict(j, d_input=ins[-1].data_ptr(), d_vis=outs[-1].data_ptr()))
     total = sum(W.cmacs(j["num_elements"], j["num_freq"], j["time_steps"]) for j in jobs)
-    with chx.Batch(descs, dict(method=method)) as b:
+    with chx.Batch(descs, dict(method=method, small_tile_elements=small)) as b:
         say(f"  uneven batch: {b.info()}")
         b.run()
         torch.cuda.synchronize()
