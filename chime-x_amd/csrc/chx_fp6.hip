@@ -38,7 +38,8 @@
 //     the kernel: each load instruction holds up the wave's in-order issue,
 //     and a DMA costs about 1.5x a direct load of the same bytes.  The half
 //     tiles at the array's right edge go two to a workgroup (one 128-row
-//     block per pair of waves, each with its own ring).
+//     block per pair of waves, each with its own ring).  The batch kernel
+//     can also run 128 x 128 tiles (each wave 64 x 64) for small arrays.
 #include "chx_internal.h"
 
 namespace chx {

@@ -102,6 +102,35 @@ def test_valid_config_without_gpu_reports_no_device():
     assert e.value.status == chx.ERR_NO_DEVICE
 
 
+def test_stream_event_and_device_helpers_host_side():
+    L = chx.lib()
+    vp = ctypes.c_void_p
+    L.chx_stream_create.argtypes = [ctypes.c_int32, ctypes.POINTER(vp)]
+    L.chx_event_create.argtypes = [ctypes.POINTER(vp)]
+    L.chx_device_alloc.argtypes = [ctypes.c_int32, ctypes.c_size_t, ctypes.POINTER(vp)]
+    for name in ("chx_stream_destroy", "chx_event_destroy", "chx_device_free"):
+        getattr(L, name).argtypes = [vp]
+    L.chx_event_record.argtypes = [vp, vp]
+    L.chx_stream_wait_event.argtypes = [vp, vp]
+    assert L.chx_stream_create(0, None) == chx.ERR_INVALID_ARGUMENT
+    assert L.chx_event_create(None) == chx.ERR_INVALID_ARGUMENT
+    assert L.chx_device_alloc(0, 16, None) == chx.ERR_INVALID_ARGUMENT
+    assert L.chx_stream_destroy(None) == chx.ERR_INVALID_ARGUMENT
+    assert L.chx_event_destroy(None) == chx.ERR_INVALID_ARGUMENT
+    assert L.chx_device_free(None) == chx.ERR_INVALID_ARGUMENT
+    assert L.chx_event_record(None, None) == chx.ERR_INVALID_ARGUMENT
+    assert L.chx_stream_wait_event(None, None) == chx.ERR_INVALID_ARGUMENT
+    assert L.chx_device_arch(0, None, 0) == chx.ERR_INVALID_ARGUMENT
+    if chx.device_count() == 0:
+        h = vp()
+        assert L.chx_stream_create(0, ctypes.byref(h)) == chx.ERR_NO_DEVICE and not h.value
+        assert L.chx_device_alloc(0, 16, ctypes.byref(h)) == chx.ERR_NO_DEVICE
+        assert chx.device_arch(0) is None
+        # the tuning cache falls back to the gfx950 key without a device
+        assert chx.job_signature([dict(num_elements=64, num_freq=1, time_steps=64)]) \
+            .startswith("gfx950/")
+
+
 def test_noalloc_refusals_are_host_side():
     # chx_correlate_noalloc refuses a request its fixed scratch cannot hold
     # and the matrix-core methods before touching a device (the pointers are
