@@ -8,7 +8,10 @@
  *   -c check_results, -v verbose, -g gen_type, -r random_seed,
  *   -p no_repeat_random, -q generate_frequency, -x/-y default_real/imag,
  *   -X/-Y initial_real/imag, -k kernel_batch (0 conference, 1 highly
- *   packed, 2 int8 matrix cores, 3 FP6 matrix cores), -h help.
+ *   packed, 2 int8 matrix cores, 3 FP6 matrix cores), -h help; and one of
+ *   its own, -o file: write the visibilities as the packed upper triangle
+ *   ([freq][N(N+1)/2][re, im] int32, reorganize_GPU_to_upper_triangle's
+ *   order, gpu_data_reorg.c:76-124) to a file.
  * Deliberate differences: -Y sets initial_imaginary (the reference writes
  * initial_real, main_wrapper.c:203-205); -k also accepts 2 and 3; byte counts are
  * 64-bit (main_wrapper.c:403); the data generator and the --check_results
@@ -52,7 +55,8 @@ static void print_help(void) {
       "  --generate_frequency (-q) [n]     Default: -1 (all). Other channels get the default value.\n"
       "  --default_real (-x) / --default_imaginary (-y) [n]  Default: 0, range [-8,7].\n"
       "  --initial_real (-X) / --initial_imaginary (-Y) [n]  Default: 0, range [-8,7].\n"
-      "  --kernel_batch (-k) [n]           Default: 0. 0 conference, 1 highly packed, 2 int8 matrix cores, 3 FP6 matrix cores.\n");
+      "  --kernel_batch (-k) [n]           Default: 0. 0 conference, 1 highly packed, 2 int8 matrix cores, 3 FP6 matrix cores.\n"
+      "  --output_triangle (-o) [file]     Write the upper-triangle visibilities ([freq][N(N+1)/2][re,im] int32).\n");
 }
 
 static int clip(int v, int off, int lo, int hi) {
@@ -131,6 +135,7 @@ int main(int argc, char **argv) {
   int T_changed = 0, without_copies = 0, check = 0, verbose = 0, gen_type = 4;
   int seed = 42, no_repeat = 0, gen_freq = -1, dre = 0, dim = 0, ire = 0, iim = 0;
   int kernel_batch = 0, U = 1;
+  const char *out_path = NULL;
   static struct option opts[] = {
       {"device", required_argument, 0, 'd'},
       {"iterations", required_argument, 0, 'i'},
@@ -151,11 +156,12 @@ int main(int argc, char **argv) {
       {"initial_real", required_argument, 0, 'X'},
       {"initial_imaginary", required_argument, 0, 'Y'},
       {"kernel_batch", required_argument, 0, 'k'},
+      {"output_triangle", required_argument, 0, 'o'},
       {"help", no_argument, 0, 'h'},
       {0, 0, 0, 0}};
   for (;;) {
     int idx = 0;
-    int c = getopt_long(argc, argv, "d:i:f:e:t:T:wcvg:r:pq:x:y:X:Y:hk:U:", opts, &idx);
+    int c = getopt_long(argc, argv, "d:i:f:e:t:T:wcvg:r:pq:x:y:X:Y:hk:U:o:", opts, &idx);
     if (c == -1) break;
     switch (c) {
       case 'h': print_help(); return 0;
@@ -194,6 +200,7 @@ int main(int argc, char **argv) {
           return -1;
         }
         break;
+      case 'o': out_path = optarg; break;
       case 'U':
         U = atoi(optarg);
         if (U < 0 || U > 1) {
@@ -297,6 +304,23 @@ int main(int argc, char **argv) {
          cmac_s / 1e12, 100.0 * 8.0 * cmac_s / INT8_PEAK_OPS, INT8_PEAK_OPS / 1e15);
 
   int rc = stages_equal ? 0 : 1;
+  if (out_path) {
+    const int64_t ntri = (int64_t)num_elem * (num_elem + 1) / 2;
+    int32_t *tri = (int32_t *)malloc(sizeof(int32_t) * 2 * ntri * num_freq);
+    if (!tri) {
+      printf("failed to allocate memory\n");
+      return 1;
+    }
+    CHECK(chx_blocks_to_triangle(num_freq, num_elem, h_out, tri));
+    FILE *fo = fopen(out_path, "wb");
+    const size_t n = (size_t)(2 * ntri * num_freq);
+    if (!fo || fwrite(tri, sizeof(int32_t), n, fo) != n) {
+      printf("failed to write %s\n", out_path);
+      rc = 1;
+    }
+    if (fo) fclose(fo);
+    free(tri);
+  }
   if (check) {
     printf("Checking results. Please wait...\n");
     double c0 = wall_s();

@@ -47,3 +47,36 @@ def test_cli_double_buffered_loop(k, w):
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "CPU matches GPU" in r.stdout and "Stages 0 and 1 agree" in r.stdout
+
+
+# Reference-generator fixtures the C host can reproduce from its command
+# line (T a multiple of 32): params are (T, F, N, gen_type, seed,
+# default_real, default_imag, initial_real, initial_imag, single_freq,
+# no_repeat), the arguments of generate_char_data_set (input_generator.c:17).
+CLI_CASES = [("c1_repeat", [0, 1, 2, 3]), ("c1_continuous", [0, 3]),
+             ("rand_cont_s7", [3]), ("single_freq_1", [1, 3])]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,methods", CLI_CASES)
+def test_cli_end_to_end_matches_reference_fixtures(golden, case, methods, tmp_path):
+    # The C host's own generator and the GPU correlation, written as the
+    # packed upper triangle (-o), equal the reference CPU correlator's output
+    # for the same generator arguments, both conventions.
+    import numpy as np
+
+    T, F, N, gen, seed, dre, dim, ire, iim, q, p = (int(v) for v in golden[f"{case}__params"])
+    for k in methods:
+        for U in (1, 0):
+            ta = 256 if T % 256 == 0 else 32
+            out = tmp_path / f"{case}_{k}_{U}.bin"
+            args = [_cli(), "-e", str(N), "-f", str(F), "-T", str(T), "-t", str(ta), "-i", "1",
+                    "-g", str(gen), "-r", str(seed), "-x", str(dre), "-y", str(dim),
+                    "-X", str(ire), "-Y", str(iim), "-q", str(q), "-k", str(k), "-U", str(U),
+                    "-o", str(out)] + (["-p"] if p else [])
+            print("  " + " ".join(args[1:]), flush=True)
+            r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+            assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+            got = np.fromfile(out, dtype=np.int32).reshape(F, N * (N + 1) // 2, 2)
+            np.testing.assert_array_equal(got, golden[f"{case}__tri_U{U}"],
+                                          err_msg=f"{case} -k {k} -U {U}")
