@@ -369,6 +369,7 @@ int plan_tables(chx_batch *b, uint8_t *base, const std::vector<int64_t> &off, Ta
       g.mode = pieces > 1 ? 2 : (r == 0 ? 0 : 1);
       g.stages_per_split = per;
       g.small = small ? 1 : 0;
+      g.corr = nullptr;
       if (pieces > 1 && r == 0) b->zero.push_back({d.d_vis, (int64_t)F * g.NB * 2048});
       gjobs.push_back(g);
     }

@@ -767,6 +767,7 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
     J.mode = kmode;
     J.stages_per_split = sps;
     J.small = 0;
+    J.corr = nullptr;
     const int64_t grid = head + tail * split;
     if (grid > 0x7FFFFFFF) return CHX_ERR_SHAPE;
     corr_f6_kernel<<<dim3((unsigned)grid), dim3(256), 0, s>>>(J, (int)head, (int)tail, tmode);

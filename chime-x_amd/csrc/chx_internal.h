@@ -15,7 +15,7 @@ struct chx_plan {
   uint32_t *d_maps;  // [2][NB]: id_x_map then id_y_map (main_wrapper.c:528-552)
   uint32_t *d_sums;  // [F][N][2] plan-owned offset sums (methods 0/1)
   int32_t ntiles;    // 128-element tiles per side (int8 matrix-core path)
-  uint8_t *d_work;   // int8 planes of one time chunk (int8 matrix-core path)
+  uint8_t *d_work;   // planes of one time chunk (matrix-core paths; int8: + code sums, corrections)
   size_t work_bytes;
   // Lazily allocated buffers of chx_correlate_host.
   uint8_t *d_in;
@@ -74,8 +74,9 @@ constexpr int kFp6Ring = 4;  // 64-sample stages in the LDS ring
 // tile ti holds ceil((ntiles - ti) / 2) tiles, the columns starting at row
 // tile ti.  A work item covers at most kMaxItemStages 64-sample stages, so
 // that every partial sum stays exact: FP6's FP32 sums (at most 128 * 64 per
-// stage) stay <= 2^24, int8's int32 sums (64x products, at most 8192 * 64
-// per stage) <= 2^30.
+// stage) stay <= 2^24, int8's int32 sums (signed 64x products, at most
+// 8192 * 64 per stage) <= 2^30, or (offset-binary code products, at most
+// 450 * 64 per stage) < 2^26.
 constexpr int kMaxItemStages = 2048;
 __host__ __device__ inline int64_t wide_tiles(int64_t ntiles) {
   const int64_t m = ntiles / 2;
@@ -126,6 +127,10 @@ struct GemmJob {
   int64_t NB;              // blocks per channel
   int F, Np, KB, ntiles, nb, U, mode, stages_per_split;  // mode 0 =, 1 +=, 2 atomic +=
   int small;  // fp6 batch only: 128 x 128 tiles (four waves of 64 x 64) instead of 128 x 256
+  // int8 plan path only (offset-binary planes): per-element offset
+  // corrections [2][F][Np] of int32 pairs, rows then columns, added by the
+  // piece that starts the chunk; nullptr elsewhere.
+  const int32_t *corr;
 };
 // work items of one channel of a GEMM job
 inline int64_t job_tiles(int fmt, const GemmJob &g) {
@@ -137,11 +142,6 @@ inline int64_t job_tiles(int fmt, const GemmJob &g) {
 struct GemmItem {
   int job, f, tile, stage0, nstages;
 };
-// [T][F][N] bytes (samples t0 .. t0+t_valid) -> planes with KB 16-sample
-// blocks (zero past t_valid and past N).
-int launch_unpack_planes(const uint8_t *in, uint8_t *planes, int N, int F, int Np,
-                         int64_t row_bytes, int64_t t0, int64_t t_valid, int KB,
-                         hipStream_t s);
 // One job's chunk for the batched unpack, starting at global workgroup
 // unit0 (increasing over jobs): units = ceil(Np/1024) * KB * F (int8) or
 // ceil(Np/1024) * ceil(2 KB / fp6_halves_per_unit(Np)) * F (fp6: one

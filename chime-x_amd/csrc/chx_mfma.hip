@@ -8,26 +8,42 @@
 //   IM = Ar.Bi^T - Ai.Br^T      (sum y_r x_i - y_i x_r)
 //   im = -IM for upper_triangle_convention 1 (cpu_corr_test.c:189),
 //        IM for 0 (cpu_corr_test.c:92).
-// No offset/preseed correction is needed and there is no time_accum cap.
+// There is no time_accum cap.
 //
 // Two passes per chunk of time:
-//  1. unpack_planes_kernel (HBM-bound): [time][freq][element] bytes ->
-//     MFMA-ready planes [plane][freq][16-sample block][element][16 B].  A
-//     byte with nibbles (R, I) is XORed with 0x88 so each nibble holds its
-//     value's 4-bit two's complement; placed in bits 6..3 of an int8 with
-//     its sign extended into bit 7 it is 8*value exactly.  Elements past N
-//     and samples past T are 0.
-//  2. corr_mfma_kernel: a workgroup = 4 wavefronts = a 128-row x 256-column
-//     tile of one channel, each wave all 128 rows x 64 columns (4x2 blocks
-//     x 2 int32 accumulators: RE = Ar.Br + Ai.Bi and IM = Ar.Bi + Ai.(-Br),
-//     -Br negated per byte in registers).  The row operands come through a
-//     ring of 4 LDS k-steps of 32 samples (8 KB each) filled by buffer->LDS
-//     DMA; each wave loads its own column operands straight into VGPRs,
-//     three k-steps ahead.
-// Every product is 64x the true one; a work item covers at most
-// kMaxItemStages 64-sample stages, so every accumulator stays a multiple of
-// 64 of magnitude <= 2^30, divided by 64 (exact) when flushed to the int32
-// output.
+//  1. unpack (HBM-bound): [time][freq][element] bytes -> MFMA-ready planes
+//     [plane][freq][16-sample block][element][16 B], elements past N and
+//     samples past T zero.  Two encodings of a nibble u (value u - 8):
+//     * plan path (unpack_u4_kernel): the offset-binary code u itself, in
+//       [0, 15].  The GEMM then accumulates code products, and -Br becomes
+//       the complement 15 - u (one XOR):
+//         RE' = sum ur_a ur_b + ui_a ui_b,  IM' = sum ur_a ui_b + ui_a (15 - ur_b),
+//       which the per-element code sums SR, SI over the chunk's n valid
+//       samples turn into the signed values (padding is code 0 and adds
+//       nothing):
+//         RE = RE' - 8 (SR_a + SI_a) + 128 n - 8 (SR_b + SI_b)
+//         IM = IM' - 8 SR_a - 7 SI_a + 8 (SR_b - SI_b).
+//       The unpack writes each workgroup's code sums, a small kernel adds
+//       them into one row and one column correction per element
+//       (u4_corrections_kernel), and the GEMM adds them as it flushes.  Codes
+//       switch four bits per operand byte and no sign bits, so the
+//       power-limited matrix cores hold a higher clock than with signed
+//       operands.
+//     * batch and serve (unpack_unit): the byte XORed with 0x88, each
+//       nibble's 4-bit two's complement placed in bits 6..3 of an int8 with
+//       its sign in bit 7: 8*value exactly; products are 64x the true ones,
+//       exact with no correction.
+//  2. corr_mfma_kernel / corr_mfma_batch_kernel: a workgroup = 4 wavefronts
+//     = a 128-row x 256-column tile of one channel, each wave all 128 rows
+//     x 64 columns (4x2 blocks x 2 int32 accumulators: RE = Ar.Br + Ai.Bi
+//     and IM = Ar.Bi + Ai.(-Br), -Br formed per byte in registers).  The
+//     row operands come through a ring of 4 LDS k-steps of 32 samples (8 KB
+//     each) filled by buffer->LDS DMA; each wave loads its own column
+//     operands straight into VGPRs, three k-steps ahead.
+// A work item covers at most kMaxItemStages 64-sample stages (131072
+// samples): signed products are multiples of 64 of magnitude <= 2^30 per
+// accumulator, divided by 64 (exact) when flushed; code products add at most
+// 450 per sample (< 2^26 per accumulator).
 #include "chx_internal.h"
 
 namespace chx {
@@ -113,6 +129,37 @@ __device__ __forceinline__ void flush_block(int32_t *blk, const v16i &re, const 
   }
 }
 
+// Flush one block of code-product accumulators (re = RE', im = IM'), adding
+// the offset corrections: rc[j] = (row RE, row IM) of register j's row, cc
+// = (column RE, column IM) of the lane's column (zeros for a piece that
+// does not start the chunk).  Same layout and modes as flush_block.
+__device__ __forceinline__ void flush_block_u4(int32_t *blk, const v16i &re, const v16i &im,
+                                               int U, int mode, int lane, const v2i (&rc)[16],
+                                               const v2i cc) {
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc(blk, (short)0, 2048 * 4, 0x00020000);
+  const int voff = ((lane >> 5) * 4 * 32 + (lane & 31)) * 8;
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    const int32_t vr = re[j] + rc[j].x + cc.x;
+    const int32_t m = im[j] + rc[j].y + cc.y;
+    const int32_t vi = U ? -m : m;
+    const int soff = (8 * (j >> 2) + (j & 3)) * 32 * 8;
+    if (mode == 0) {
+      v2i v = {vr, vi};
+      __builtin_amdgcn_raw_buffer_store_b64(v, rsrc, voff, soff, 0);
+    } else if (mode == 1) {
+      v2i a = __builtin_amdgcn_raw_buffer_load_b64(rsrc, voff, soff, 0);
+      a.x += vr;
+      a.y += vi;
+      __builtin_amdgcn_raw_buffer_store_b64(a, rsrc, voff, soff, 0);
+    } else {
+      __builtin_amdgcn_raw_ptr_buffer_atomic_add_i32(vr, rsrc, voff, soff, 0);
+      __builtin_amdgcn_raw_ptr_buffer_atomic_add_i32(vi, rsrc, voff + 4, soff, 0);
+    }
+  }
+}
+
 // Pass 1.  Workgroup = 256 element quads (1024 elements) x 16 samples of
 // one channel.  Thread: 16 word loads (one per sample row; 256 contiguous
 // bytes per wave and row), four 4x4 byte transposes, plane split; the
@@ -168,14 +215,135 @@ __device__ __forceinline__ void unpack_unit(const uint8_t *__restrict__ in,
   }
 }
 
-// Grid: x = channel, y = element group, z = block, so that workgroups
-// running together read whole input rows.
-__global__ __launch_bounds__(256) void unpack_planes_kernel(
-    const uint8_t *__restrict__ in, uint8_t *__restrict__ out_planes, int N, int F, int Np,
-    int64_t row_bytes, int64_t t0, int64_t t_valid, int KB) {
+// Pass 1 of the plan path: offset-binary code planes.  Workgroup = 256
+// element quads (1024 elements) of one channel x the 16-sample blocks run,
+// run + R, run + 2R, ... of the chunk (R = gridDim.z runs); grid x =
+// channel, y = element group, z = run.  About kU4Workgroups workgroups in
+// all, so that at any time the resident ones unpack neighbouring blocks of
+// every channel, as a grid of one block per workgroup would (workgroups
+// running together read whole input rows and write neighbouring plane
+// rows).  Per block, as unpack_unit: 16 word loads per thread (the next
+// block's issued before this one is written), four 4x4 byte transposes,
+// the codes split into the R and I planes, LDS staging, full-run stores.
+// Each thread also adds up its four elements' R and I codes over its
+// blocks and stores them as one word per element (SR | SI << 16; at most
+// 16 * 15 * kU4MaxBlocks = 61440) into partials[run][f][element].  Samples
+// past t_valid and elements past N are code 0.
+constexpr int kU4Workgroups = 1024;
+constexpr int kU4MaxBlocks = 256;  // blocks per workgroup: the partials fit 16 bits
+__host__ __device__ inline int64_t u4_runs(int64_t KB, int64_t F, int64_t Np) {
+  const int64_t per = F * ((Np + 1023) / 1024);
+  int64_t r = (kU4Workgroups + per - 1) / per;
+  const int64_t need = (KB + kU4MaxBlocks - 1) / kU4MaxBlocks;
+  if (r < need) r = need;
+  return r < KB ? r : KB;
+}
+__global__ __launch_bounds__(256) void unpack_u4_kernel(
+    const uint8_t *__restrict__ in, uint8_t *__restrict__ out_planes,
+    uint32_t *__restrict__ partials, int N, int F, int Np, int64_t row_bytes, int64_t t0,
+    int64_t t_valid, int KB) {
   __shared__ __attribute__((aligned(16))) v4u stage[2][1024 + 64];  // [plane][element], padded
-  unpack_unit(in, out_planes, N, F, Np, row_bytes, t0, t_valid, KB, blockIdx.y, blockIdx.z, 0,
-              blockIdx.x, stage);
+  const int f = blockIdx.x, qb = blockIdx.y, run = blockIdx.z, R = gridDim.z;
+  const int tid = threadIdx.x;
+  const int q = qb * 256 + tid;  // element quad
+  // Buffer loads and stores, branch-free: a block's descriptor covers its
+  // valid sample rows only, and lanes past N (or past the block's last
+  // element) use an offset beyond every descriptor, so they read zeros
+  // and their stores are dropped.  Streamed once: non-temporal (aux 2).
+  const int rb = (int)row_bytes;  // 16 rows of a block fit in an int
+  const int voff_in = 4 * q < N ? f * N + 4 * q : 0x7FFFFFF0;
+  auto load = [&](uint32_t (&w)[16], int kb) {
+    const int64_t t = 16 * (int64_t)kb;
+    const int64_t rows = t_valid - t < 0 ? 0 : (t_valid - t > 16 ? 16 : t_valid - t);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(in + (t0 + t) * row_bytes), (short)0, (int)(rows * row_bytes), 0x00020000);
+#pragma unroll
+    for (int s = 0; s < 16; s++)
+      w[s] = __builtin_amdgcn_raw_buffer_load_b32(rs, voff_in + s * rb, 0, 2);
+  };
+  const int64_t plane_stride = (int64_t)F * KB * Np * 16;
+  const int e_base = qb * 1024;
+  const int n_el = Np - e_base < 1024 ? Np - e_base : 1024;
+  uint32_t sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
+  uint32_t w[16];
+  load(w, run);
+  for (int kb = run; kb < KB; kb += R) {
+    uint32_t c[4][4];  // [element][4-sample group]
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+      transpose4(w[4 * m], w[4 * m + 1], w[4 * m + 2], w[4 * m + 3], c[0][m], c[1][m], c[2][m],
+                 c[3][m]);
+    if (kb + R < KB) load(w, kb + R);
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      uint32_t r[4], i[4];
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        r[m] = (c[e][m] >> 4) & 0x0F0F0F0Fu;
+        i[m] = c[e][m] & 0x0F0F0F0Fu;
+      }
+      // bytes <= 15: four words add bytewise without carries (<= 60)
+      sr[e] = __builtin_amdgcn_sad_u8(r[0] + r[1] + r[2] + r[3], 0u, sr[e]);
+      si[e] = __builtin_amdgcn_sad_u8(i[0] + i[1] + i[2] + i[3], 0u, si[e]);
+      const int el = 4 * tid + e;
+      const int slot = el + (el >> 4);  // one pad chunk per 16: conflict-free b128 writes
+      stage[0][slot] = (v4u){r[0], r[1], r[2], r[3]};
+      stage[1][slot] = (v4u){i[0], i[1], i[2], i[3]};
+    }
+    __syncthreads();
+    uint8_t *dst = out_planes + (((int64_t)f * KB + kb) * Np + e_base) * 16;
+    const __amdgpu_buffer_rsrc_t d0 =
+        __builtin_amdgcn_make_buffer_rsrc(dst, (short)0, n_el * 16, 0x00020000);
+    const __amdgpu_buffer_rsrc_t d1 =
+        __builtin_amdgcn_make_buffer_rsrc(dst + plane_stride, (short)0, n_el * 16, 0x00020000);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int el = k * 256 + tid;
+      const int slot = el + (el >> 4);
+      __builtin_amdgcn_raw_buffer_store_b128(stage[0][slot], d0, el * 16, 0, 2);
+      __builtin_amdgcn_raw_buffer_store_b128(stage[1][slot], d1, el * 16, 0, 2);
+    }
+    __syncthreads();
+  }
+  if (4 * q < Np) {
+    const v4u v = {sr[0] | si[0] << 16, sr[1] | si[1] << 16, sr[2] | si[2] << 16,
+                   sr[3] | si[3] << 16};
+    *(v4u *)(partials + ((int64_t)run * F + f) * Np + 4 * q) = v;
+  }
+}
+
+// The offset corrections of one chunk (n valid samples): per channel f and
+// element e, SR and SI = its code sums over the chunk (the nrun partials),
+// then corr[0][f][e] = (-8 (SR + SI) + 128 n, -8 SR - 7 SI) for its rows and
+// corr[1][f][e] = (-8 (SR + SI), 8 (SR - SI)) for its columns.  Workgroup =
+// 64 elements x 4 run quarters (lanes of a wave read 64 consecutive
+// partials), added up through LDS.
+__global__ __launch_bounds__(256) void u4_corrections_kernel(const uint32_t *__restrict__ partials,
+                                                             int32_t *__restrict__ corr, int F,
+                                                             int Np, int nrun, int32_t n) {
+  __shared__ uint32_t part[2][4][64];
+  const int64_t FN = (int64_t)F * Np;
+  const int lane = threadIdx.x & 63, quarter = threadIdx.x >> 6;
+  const int64_t i = (int64_t)blockIdx.x * 64 + lane;  // (f, e) flattened
+  uint32_t sr = 0, si = 0;
+  if (i < FN) {
+    for (int r = quarter; r < nrun; r += 4) {
+      const uint32_t v = partials[(int64_t)r * FN + i];
+      sr += v & 0xFFFFu;
+      si += v >> 16;
+    }
+  }
+  part[0][quarter][lane] = sr;
+  part[1][quarter][lane] = si;
+  __syncthreads();
+  if (quarter == 0 && i < FN) {
+    const int32_t SR = (int32_t)(part[0][0][lane] + part[0][1][lane] + part[0][2][lane] +
+                                 part[0][3][lane]);
+    const int32_t SI = (int32_t)(part[1][0][lane] + part[1][1][lane] + part[1][2][lane] +
+                                 part[1][3][lane]);
+    ((v2i *)corr)[i] = (v2i){-8 * (SR + SI) + 128 * n, -8 * SR - 7 * SI};
+    ((v2i *)corr)[FN + i] = (v2i){-8 * (SR + SI), 8 * (SR - SI)};
+  }
 }
 
 // Batch: one launch unpacks the chunks of many jobs.  Workgroup g belongs
@@ -236,13 +404,15 @@ __device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti_a, int &
 //          tile; the four waves share the rows);
 //    PAIR: two half tiles of column tile ct: waves 0-1 rows from kTile
 //          ti_a, waves 2-3 rows from kTile ti_b, columns kTile ct + 64 (w%2);
-// RE += Ar.Br + Ai.Bi,  IM += Ar.Bi + Ai.(-Br),  where -Br is negated per
-// byte in registers (exact: every byte is 8 x value, in [-64, 56]).  The
+// RE += Ar.Br + Ai.Bi,  IM += Ar.Bi + Ai.(-Br),  where -Br is formed per
+// byte in registers: !U4 (8 x value bytes, in [-64, 56]) the exact
+// negation; U4 (offset-binary codes, in [0, 15]) the complement 15 - u,
+// and the flush adds the offset corrections J.corr.  The
 // row operands, shared by the four waves, come through the LDS ring by
 // buffer->LDS DMA; each wave loads its own column operands straight into
 // VGPRs, three k-steps ahead, so every operand byte crosses the CU's load
 // path once.
-template <bool PAIR>
+template <bool PAIR, bool U4 = false>
 __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const int ti_a,
                                           const int ti_b, const int ct, const int st_begin,
                                           const int nst, uint8_t (*lds)[kSlotBytes]) {
@@ -327,9 +497,16 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
   auto step_ops = [&](const v4i &br0, const v4i &br1, const v4i &bi0, const v4i &bi1,
                       int nslot) {
     const v4i br[2] = {br0, br1}, bi[2] = {bi0, bi1};
-    // -Br per byte (bytes are multiples of 8): IM += Ar.Bi + Ai.(-Br)
-    const v4i nbr[2] = {((br0 ^ (int)0xF8F8F8F8u) + 0x08080808) & (int)0xF8F8F8F8u,
-                        ((br1 ^ (int)0xF8F8F8F8u) + 0x08080808) & (int)0xF8F8F8F8u};
+    // -Br per byte: IM += Ar.Bi + Ai.(-Br).  Signed bytes (multiples of 8)
+    // are negated exactly; codes u are complemented to 15 - u.
+    v4i nbr[2];
+    if constexpr (U4) {
+      nbr[0] = br0 ^ 0x0F0F0F0F;
+      nbr[1] = br1 ^ 0x0F0F0F0F;
+    } else {
+      nbr[0] = ((br0 ^ (int)0xF8F8F8F8u) + 0x08080808) & (int)0xF8F8F8F8u;
+      nbr[1] = ((br1 ^ (int)0xF8F8F8F8u) + 0x08080808) & (int)0xF8F8F8F8u;
+    }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const v4i xr = i == 3 ? ar3n : ar[i], xi = i == 3 ? ai3n : ai[i];
@@ -435,18 +612,58 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
       __builtin_amdgcn_sched_barrier(0);
     }
   }
-  // Every partial sum is a multiple of 64 of magnitude <= 2 * 64 * 64 *
-  // 32 * (2 * kMaxItemStages) = 2^30: no int32 overflow.
+  // Signed operands: every partial sum is a multiple of 64 of magnitude <=
+  // 2 * 64 * 64 * 32 * (2 * kMaxItemStages) = 2^30.  Codes: <= 450 per
+  // sample, < 2^26.  No int32 overflow either way.
+  if constexpr (U4) {
+    // Corrections: the piece that starts the chunk adds them, once.
+    const bool first = st_begin == 0;
+    const v2i *rowc = (const v2i *)J.corr + (int64_t)f * Np;
+    const v2i *colc = (const v2i *)J.corr + ((int64_t)F + f) * Np;
+    v2i cc[2];
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 2; j++)
+      cc[j] = first ? colc[(bx0 + j) * 32 + (lane & 31)] : (v2i){0, 0};
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-      const int by = by0 + i, bx = bx0 + j;
-      if (bx >= by && bx < nb) {
-        const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
-        flush_block(vis + ((int64_t)f * NB + b) * 2048, re[i][j], im[i][j], U, mode, lane);
+    for (int i = 0; i < 4; i++) {
+      const int by = by0 + i;
+      // rows of register j: 32 by + 8 (j/4) + 4 (lane/32) + j%4
+      v2i rc[16];
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        v4i lo = {0, 0, 0, 0}, hi = {0, 0, 0, 0};
+        if (first) {
+          const v2i *p = rowc + by * 32 + 8 * g + 4 * (lane >> 5);
+          lo = *(const v4i *)p;
+          hi = *(const v4i *)(p + 2);
+        }
+        rc[4 * g] = (v2i){lo.x, lo.y};
+        rc[4 * g + 1] = (v2i){lo.z, lo.w};
+        rc[4 * g + 2] = (v2i){hi.x, hi.y};
+        rc[4 * g + 3] = (v2i){hi.z, hi.w};
+      }
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const int bx = bx0 + j;
+        if (bx >= by && bx < nb) {
+          const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
+          flush_block_u4(vis + ((int64_t)f * NB + b) * 2048, re[i][j], im[i][j], U, mode, lane,
+                         rc, cc[j]);
+        }
       }
     }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const int by = by0 + i, bx = bx0 + j;
+        if (bx >= by && bx < nb) {
+          const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
+          flush_block(vis + ((int64_t)f * NB + b) * 2048, re[i][j], im[i][j], U, mode, lane);
+        }
+      }
+  }
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
@@ -475,9 +692,9 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(GemmJob J, int nst, i
   int ti_a, ti_b, ct;
   tile_coords((int)(item % per_f), J.ntiles, ti_a, ti_b, ct);
   if (ti_b >= 0 && ti_b != ti_a)
-    gemm_tile<true>(J, (int)(item / per_f), ti_a, ti_b, ct, s0, ns, lds);
+    gemm_tile<true, true>(J, (int)(item / per_f), ti_a, ti_b, ct, s0, ns, lds);
   else
-    gemm_tile<false>(J, (int)(item / per_f), ti_a, ti_a, ct, s0, ns, lds);
+    gemm_tile<false, true>(J, (int)(item / per_f), ti_a, ti_a, ct, s0, ns, lds);
 }
 
 // Batch: persistent workgroups take work items from 8 per-XCD queues: a
@@ -540,9 +757,20 @@ static int64_t chunk_samples(const chx_plan *p) {
   return tc;
 }
 
+// The workspace of one chunk: code planes [2][F][KB][Np][16 B], then the
+// unpack's code-sum partials [run][F][Np] u32, then the offset corrections
+// [2][F][Np] int32 pairs.
+static int64_t u4_plane_bytes(const chx_plan *p) {
+  return 2 * (int64_t)p->cfg.num_freq * p->ntiles * kTile * chunk_samples(p);
+}
+static int64_t u4_partial_bytes(const chx_plan *p) {
+  const int64_t Np = (int64_t)p->ntiles * kTile;
+  return u4_runs(chunk_samples(p) / 16, p->cfg.num_freq, Np) * p->cfg.num_freq * Np * 4;
+}
+
 size_t mfma_workspace_bytes(const chx_plan *p) {
   const int64_t Np = (int64_t)p->ntiles * kTile;
-  return (size_t)(2 * (int64_t)p->cfg.num_freq * Np * chunk_samples(p));
+  return (size_t)(u4_plane_bytes(p) + u4_partial_bytes(p) + 2 * (int64_t)p->cfg.num_freq * Np * 8);
 }
 
 int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode,
@@ -558,6 +786,10 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
   if (cus < 1) cus = 1;
   const int64_t tc = chunk_samples(p);
   const int64_t row_bytes = (int64_t)F * N;
+  if (row_bytes > 0x7FFFFFFF / 16) {  // the unpack addresses 16 input rows with 32-bit offsets
+    set_error_detail("int8 matrix-core method: num_freq * num_elements must be below 2^27");
+    return CHX_ERR_SHAPE;
+  }
   const int ring = kMfmaRing;  // stage counts are multiples of it
   for (int64_t t0 = 0; t0 < T; t0 += tc) {
     const int64_t tv = T - t0 < tc ? T - t0 : tc;  // valid samples
@@ -601,11 +833,21 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
       p->prof_ev->insert(p->prof_ev->end(), ev, ev + 3);
       CHX_HIP(hipEventRecord(ev[0], s));
     }
-    int rc0 = launch_unpack_planes(in, p->d_work, N, F, Np, row_bytes, t0, tv, KB, s);
-    if (rc0) return rc0;
+    uint32_t *partials = (uint32_t *)(p->d_work + u4_plane_bytes(p));
+    int32_t *corr = (int32_t *)(p->d_work + u4_plane_bytes(p) + u4_partial_bytes(p));
+    const int64_t nrun = u4_runs(KB, F, Np);
+    unpack_u4_kernel<<<dim3((unsigned)F, (unsigned)((Np + 1023) / 1024), (unsigned)nrun),
+                       dim3(256), 0, s>>>(in, p->d_work, partials, N, F, Np, row_bytes, t0, tv,
+                                          KB);
+    CHX_HIP(hipGetLastError());
+    const int64_t FN = (int64_t)F * Np;
+    u4_corrections_kernel<<<dim3((unsigned)((FN + 63) / 64)), dim3(256), 0, s>>>(
+        partials, corr, F, Np, (int)nrun, (int32_t)tv);
+    CHX_HIP(hipGetLastError());
     if (p->prof) CHX_HIP(hipEventRecord(ev[1], s));
     GemmJob J;
     J.planes = p->d_work;
+    J.corr = corr;
     J.planes1 = nullptr;
     J.vis = vis;
     J.NB = p->NB;
@@ -625,15 +867,6 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
     CHX_HIP(hipGetLastError());
     if (p->prof) CHX_HIP(hipEventRecord(ev[2], s));
   }
-  return CHX_SUCCESS;
-}
-
-int launch_unpack_planes(const uint8_t *in, uint8_t *planes, int N, int F, int Np,
-                         int64_t row_bytes, int64_t t0, int64_t t_valid, int KB,
-                         hipStream_t s) {
-  unpack_planes_kernel<<<dim3((unsigned)F, (unsigned)((Np + 1023) / 1024), (unsigned)KB),
-                         dim3(256), 0, s>>>(in, planes, N, F, Np, row_bytes, t0, t_valid, KB);
-  CHX_HIP(hipGetLastError());
   return CHX_SUCCESS;
 }
 

@@ -193,6 +193,7 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
     g.mode = pieces > 1 ? 2 : 1;  // accumulate (atomically when a tile is cut in time)
     g.stages_per_split = nst;
     g.small = 0;
+    g.corr = nullptr;
     gjobs.push_back(g);
   }
   std::vector<GemmItem> items;

@@ -160,6 +160,23 @@ def test_mfma_long_integration_saturation_closed_form(method):
 
 
 @pytest.mark.parametrize("method", MATRIX)
+@pytest.mark.parametrize("U", [1, 0])
+def test_mfma_every_code_pair_over_a_full_item(method, U):
+    # Element e holds byte e at every sample, so the 256 elements cover
+    # every (R, I) code pair, including (7,7) and (-8,-8), and every pair
+    # of elements meets in one block; over 131072 samples (one work item of
+    # the most stages) the int8 path's offset-binary code products reach
+    # their largest sums.  Constant data: the result is T/32 times the
+    # oracle's over 32 samples.
+    T, F, N = 131072, 1, 256
+    row = np.arange(N, dtype=np.uint8)
+    data = np.broadcast_to(row, (T, F, N)).copy()
+    want = O.correlate_blocks(np.ascontiguousarray(data[:32]), U).astype(np.int64) * (T // 32)
+    got = run(data, method, U, 256)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("method", MATRIX)
 def test_mfma_time_split_and_linearity(method):
     # Small N with long T takes the split-time (atomic) path.
     T, F, N = 16384, 1, 64
