@@ -1,26 +1,33 @@
 // chx_fp6.hip -- the correlator on CDNA4 FP6 matrix cores, exactly.
 //
-// The 4-bit values v = nibble - 8 in [-8, 7] are carried as v/8 in the FP6
-// E2M3 format (1 sign, 2 exponent bits with bias 1, 3 mantissa bits): |v| =
-// 0..7 are the subnormals m/8 (mantissa m = |v|) and |v| = 8 is 1.0
-// (exponent 1, mantissa 0), so every code is simply sign<<5 | |v|.
-// v_mfma_scale_f32_32x32x64_f8f6f4 with both operands E2M3 and a block
-// scale of 2^3 on each issues a 32x32x64 product of the integers v in the
-// cycles an int8 32x32x32 one takes, and accumulates in FP32.  (The same
-// values as E3M2 integers, all normal numbers with scales 2^0, run 5%
-// slower: on random data the matrix cores are power-limited, and the
-// subnormal operands hold a higher clock.)  Per 32x32 block two
+// v_mfma_scale_f32_32x32x64_f8f6f4 with both operands E2M3 (1 sign, 2
+// exponent bits with bias 1, 3 mantissa bits) and a block scale of 2^3 on
+// each issues a 32x32x64 product of small integers in the cycles an int8
+// 32x32x32 one takes, and accumulates in FP32.  Per 32x32 block two
 // accumulators:
-//   RE = Ar.Br^T + Ai.Bi^T,  IM = Ar.Bi^T + Ai.(-Br)^T,  im = U ? -IM : IM,
-// where -Br is Br with the sign bit of every code flipped (exact).  Every
-// product is an integer of magnitude <= 64, so a sample adds at most 128 to
-// an accumulator; a work item covers at most kMaxItemStages = 2048 stages of
-// 64 samples, so every partial sum is an integer of magnitude <= 2^24 and
-// exact in FP32: the int32 visibilities are bit-identical to the int8
-// path's.
+//   RE = Ar.Br^T + Ai.Bi^T,  IM = Ar.Bi^T + Ai.(-Br)^T,  im = U ? -IM : IM.
+// Two encodings of a nibble u (value v = u - 8):
+//  * plan path: u/8, whose E2M3 code is u itself (u < 8: the subnormal u/8;
+//    u >= 8: exponent 1, mantissa u - 8); -Br is the complement 15 - u.  The
+//    accumulators hold code products (the same RE', IM' and offset
+//    corrections as the int8 plan path, chx_mfma.hip), at most 450 per
+//    sample: they start at -2^24 and a work item covers at most
+//    kMaxStagesU4 = 1024 stages of 64 samples, so every partial sum is an
+//    integer of magnitude <= 2^24, exact in FP32.  Codes switch four bits
+//    and every product is >= 0, so the power-limited matrix cores hold a
+//    higher clock than with signed values.
+//  * batch and serve: v/8, |v| = 0..7 the subnormals m/8 and |v| = 8 1.0, so
+//    every code is sign<<5 | |v|; -Br flips the sign bits.  Every product is
+//    an integer of magnitude <= 64, a sample adds at most 128, and a work
+//    item covers at most kMaxItemStages = 2048 stages: partial sums <= 2^24.
+// (The signed values as E3M2 integers, all normal numbers with scales 2^0,
+// run 5% slower: on random data the matrix cores are power-limited, and the
+// subnormal operands hold a higher clock.)  Either way the int32
+// visibilities are bit-identical to the int8 path's.
 //
 // Two passes per time chunk:
-//  1. unpack_f6_kernel (HBM-bound): [time][freq][element] bytes -> E2M3
+//  1. unpack_f6_u4_kernel / unpack_f6_batch_kernel (HBM-bound):
+//     [time][freq][element] bytes -> E2M3
 //     codes, bit-packed 32 per 24 bytes (code j at bits 6j..6j+5, little
 //     endian), as two arrays per plane: P0 = bytes 0-15 and P1 = bytes 16-23
 //     of each element's 32-sample half of a 64-sample block:
@@ -57,12 +64,16 @@ constexpr int kTile = kMfmaTile;     // 128: tile rows; a tile has 2 * kTile col
 constexpr int kK = 64;               // samples per k-step (= per stage)
 constexpr int kRing = kFp6Ring;      // LDS stages in the ring
 constexpr int kMaxStages = kMaxItemStages;  // stages of one work item
+// stages of one work item on offset-binary codes (the plan path): 450 * 64
+// * 1024 = 29491200 < 2^25, so from -2^24 every partial sum stays <= 2^24
+constexpr int kMaxStagesU4 = 1024;
 constexpr int kRowGroups = 4;        // 32-element row groups per stage (in LDS)
 constexpr int kP0Bytes = 2 * kRowGroups * 1024;  // [plane 2][group 4][64 lanes][16 B]
 constexpr int kP1Bytes = 2 * kRowGroups * 512;   // [plane 2][group 4][64 lanes][8 B]
 constexpr int kSlotBytes = kP0Bytes + kP1Bytes;  // one 128-row block of one stage
 constexpr int kRingBytes = kRing * kSlotBytes;   // a ring of one row block
-static_assert(kMaxStages % kRing == 0, "work items hold whole rings");
+static_assert(kMaxStages % kRing == 0 && kMaxStagesU4 % kRing == 0,
+              "work items hold whole rings");
 
 // MFMA operand format and block scale: E2M3 (format 2 of the f8f6f4
 // instruction) holding v/8, scaled back by the E8M0 exponent 127 + 3.
@@ -189,25 +200,6 @@ __device__ __forceinline__ void unpack_f6_unit(const uint8_t *__restrict__ in,
   }
 }
 
-// Pass 1 over one plan chunk.  Grid: x = channel, y = element group, z =
-// unit of halves (fp6_halves_per_unit(Np) each), so that workgroups running
-// together read whole input rows.
-__global__ __launch_bounds__(256) void unpack_f6_kernel(
-    const uint8_t *__restrict__ in, uint8_t *__restrict__ p0, uint8_t *__restrict__ p1, int N,
-    int F, int Np, int64_t row_bytes, int64_t t0, int64_t t_valid, int KB) {
-  __shared__ __attribute__((aligned(16))) v4u s0[1024 + 64];  // [element] P0, padded
-  __shared__ __attribute__((aligned(16))) v2u s1[1024 + 64];  // [element] P1, padded
-  const int g = fp6_halves_per_unit(Np);
-  const int hk0 = blockIdx.z * g;
-  const int nh = 2 * KB - hk0 < g ? 2 * KB - hk0 : g;
-  if (Np >= 1024)
-    unpack_f6_unit<false>(in, p0, p1, N, F, Np, row_bytes, t0, t_valid, KB, 0, blockIdx.y, hk0,
-                          1, blockIdx.x, s0, s1);
-  else
-    unpack_f6_unit<true>(in, p0, p1, N, F, Np, row_bytes, t0, t_valid, KB, 0, 0, hk0, nh,
-                         blockIdx.x, s0, s1);
-}
-
 // Batch: one launch unpacks the chunks of many jobs; workgroup g belongs to
 // the job j with jobs[j].unit0 <= g < jobs[j+1].unit0 (binary search); its
 // unit within the job is (unit of halves, element group, channel), channel
@@ -240,6 +232,151 @@ __global__ __launch_bounds__(256) void unpack_f6_batch_kernel(const UnpackJob *_
                          J.t_valid, J.KB_dst, J.kb_dst0, 0, hk0, nh, f, s0, s1);
 }
 
+// Pass 1 of the plan path: offset-binary codes.  The nibble u (value u - 8)
+// is carried as u/8, whose E2M3 code is u itself (u < 8: the subnormal u/8;
+// u >= 8: exponent 1, mantissa u - 8), so this unpack only transposes and
+// packs the nibbles.  A unit is 1024 elements x one 32-sample half, or for
+// an array under 1024 elements fp6_halves_per_unit(Np) consecutive halves of
+// all its elements (as unpack_f6_unit).  Grid x = channel, y = element
+// group, z = run: workgroup `run` takes the units run, run + R, ... of the
+// chunk (R = gridDim.z, about kF6U4Workgroups workgroups in all), so that
+// the resident workgroups work on neighbouring units at any time.  Loads and
+// stores go through buffer descriptors (branch-free: lanes past N or past
+// the valid samples read zeros, stores past the unit are dropped),
+// non-temporal; the workgroups resident on a CU overlap each other's loads
+// and stores.
+// Besides the planes each workgroup writes its elements' code sums over its
+// units, SR | SI << 16 (at most 15 * 32 * 136 = 65280 with at most 136
+// halves per element and workgroup), into partials[run][f][element].
+// Samples past t_valid and elements past N are code 0.
+constexpr int kF6U4Workgroups = 1024;
+constexpr int kF6U4MaxHalves = 136;
+__host__ __device__ inline int64_t f6_u4_units(int64_t KB, int Np) {
+  const int hpu = fp6_halves_per_unit(Np);
+  return (2 * KB + hpu - 1) / hpu;
+}
+__host__ __device__ inline int64_t f6_u4_runs(int64_t KB, int64_t F, int Np) {
+  const int64_t units = f6_u4_units(KB, Np);
+  const int64_t per = F * ((Np + 1023) / 1024);
+  int64_t r = (kF6U4Workgroups + per - 1) / per;
+  const int64_t upw = kF6U4MaxHalves / fp6_halves_per_unit(Np);  // units per workgroup
+  const int64_t need = (units + upw - 1) / upw;
+  if (r < need) r = need;
+  return r < units ? r : units;
+}
+__global__ __launch_bounds__(256) void unpack_f6_u4_kernel(
+    const uint8_t *__restrict__ in, uint8_t *__restrict__ p0, uint8_t *__restrict__ p1,
+    uint32_t *__restrict__ partials, int N, int F, int Np, int64_t row_bytes, int64_t t0,
+    int64_t t_valid, int KB) {
+  __shared__ __attribute__((aligned(16))) v4u s0[1024 + 64];  // [element] P0, padded
+  __shared__ __attribute__((aligned(16))) v2u s1[1024 + 64];  // [element] P1, padded
+  const int f = blockIdx.x, qb = blockIdx.y, run = blockIdx.z, R = gridDim.z;
+  const int tid = threadIdx.x;
+  const bool multi = Np < 1024;
+  const int hpu = fp6_halves_per_unit(Np);
+  const int epq = multi ? Np / 4 : 256;  // element quads per half
+  const int sub = multi ? tid / epq : 0;  // which half of the unit
+  const int q = multi ? tid % epq : qb * 256 + tid;  // element quad
+  const bool lane_ok = sub < hpu && 4 * q < N;
+  const int rb = (int)row_bytes;  // the rows of a unit fit 32-bit offsets
+  // lanes that read nothing: an offset past every descriptor (the row
+  // offsets added below keep it there, modulo 2^32)
+  const uint32_t voff = lane_ok ? (uint32_t)(32 * sub * rb + f * N + 4 * q) : 0x80000000u;
+  const int64_t units = f6_u4_units(KB, Np);
+  const int e_base = qb * 1024;
+  auto load = [&](uint32_t (&w)[32], int64_t u) {
+    // the row offsets are formed here, per unit, not kept live across the
+    // loop in 32 registers
+    uint32_t vo = voff;
+    asm volatile("" : "+v"(vo));
+    const int64_t hk0 = u * hpu;
+    const int64_t nh = 2 * (int64_t)KB - hk0 < hpu ? 2 * (int64_t)KB - hk0 : hpu;
+    const int64_t t = 32 * hk0, span = 32 * nh;
+    const int64_t rows = t_valid - t < 0 ? 0 : (t_valid - t > span ? span : t_valid - t);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(in + (t0 + t) * row_bytes), (short)0, (int)(rows * row_bytes), 0x00020000);
+#pragma unroll
+    for (int s = 0; s < 32; s++)
+      w[s] = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)(vo + (uint32_t)(s * rb)), 0, 2);
+  };
+  uint32_t sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
+  for (int64_t u = run; u < units; u += R) {
+    const int64_t hk0 = u * hpu;
+    const int nh = 2 * (int64_t)KB - hk0 < hpu ? (int)(2 * (int64_t)KB - hk0) : hpu;
+    uint32_t c[4][8];  // [element][4-sample group]
+    {
+      uint32_t w[32];
+      load(w, u);
+#pragma unroll
+      for (int m = 0; m < 8; m++)
+        transpose4(w[4 * m], w[4 * m + 1], w[4 * m + 2], w[4 * m + 3], c[0][m], c[1][m],
+                   c[2][m], c[3][m]);
+    }
+    const int n_el = multi ? nh * Np : (Np - e_base < 1024 ? Np - e_base : 1024);
+    const int el0 = multi ? sub * Np + 4 * q : 4 * tid;  // this thread's entries
+#pragma unroll
+    for (int pl = 0; pl < 2; pl++) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        uint32_t code[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++)
+          code[m] = pl == 0 ? (c[e][m] >> 4) & 0x0F0F0F0Fu : c[e][m] & 0x0F0F0F0Fu;
+        // bytes <= 15: eight words add bytewise without carries (<= 120)
+        const uint32_t bs = code[0] + code[1] + code[2] + code[3] + code[4] + code[5] +
+                            code[6] + code[7];
+        if (pl == 0)
+          sr[e] = __builtin_amdgcn_sad_u8(bs, 0u, sr[e]);
+        else
+          si[e] = __builtin_amdgcn_sad_u8(bs, 0u, si[e]);
+        const int el = el0 + e;
+        const int slot = el + (el >> 4);  // one pad entry per 16
+        v4u a;
+        v2u b;
+        pack32(code, a, b);
+        if (el < 1024) {
+          s0[slot] = a;
+          s1[slot] = b;
+        }
+      }
+      __syncthreads();
+      // (plane, channel, destination half) of the unit's first half
+      const int64_t row = ((int64_t)pl * F + f) * 2 * KB + hk0;
+      const __amdgpu_buffer_rsrc_t d0 = __builtin_amdgcn_make_buffer_rsrc(
+          p0 + (row * Np + e_base) * 16, (short)0, n_el * 16, 0x00020000);
+      const __amdgpu_buffer_rsrc_t d1 = __builtin_amdgcn_make_buffer_rsrc(
+          p1 + (row * Np + e_base) * 8, (short)0, n_el * 8, 0x00020000);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int el = k * 256 + tid;
+        const int slot = el + (el >> 4);
+        __builtin_amdgcn_raw_buffer_store_b128(s0[slot], d0, el * 16, 0, 2);
+        __builtin_amdgcn_raw_buffer_store_b64(s1[slot], d1, el * 8, 0, 2);
+      }
+      __syncthreads();
+    }
+  }
+  uint32_t *dst = partials + ((int64_t)run * F + f) * Np;
+  if (!multi) {
+    if (4 * q < Np)
+      *(v4u *)(dst + 4 * q) = (v4u){sr[0] | si[0] << 16, sr[1] | si[1] << 16,
+                                    sr[2] | si[2] << 16, sr[3] | si[3] << 16};
+  } else {
+    // the halves of a unit hold the same elements: add them up through LDS
+    uint32_t *red = (uint32_t *)&s0[0];  // [half][element]
+    if (sub < hpu) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) red[sub * Np + 4 * q + e] = sr[e] | si[e] << 16;
+    }
+    __syncthreads();
+    for (int e = tid; e < Np; e += 256) {
+      uint32_t v = 0;
+      for (int h = 0; h < hpu; h++) v += red[h * Np + e];
+      dst[e] = v;
+    }
+  }
+}
+
 // Flush one block's FP32 accumulators (exact integers) into int32
 // visibilities.  Lane l owns column n = l%32 and rows m = 8*(j/4) +
 // 4*(l/32) + j%4 of register j.  re holds Ar.Br + Ai.Bi, im holds Ar.Bi -
@@ -253,6 +390,38 @@ __device__ __forceinline__ void flush_block_f32(int32_t *blk, const v16f &re, co
   for (int j = 0; j < 16; j++) {
     const int32_t vr = (int32_t)re[j];  // an integer of magnitude <= 2^24: exact
     const int32_t vi = (int32_t)(U ? -im[j] : im[j]);
+    const int soff = (8 * (j >> 2) + (j & 3)) * 32 * 8;
+    if (mode == 0) {
+      v2i v = {vr, vi};
+      __builtin_amdgcn_raw_buffer_store_b64(v, rsrc, voff, soff, 0);
+    } else if (mode == 1) {
+      v2i a = __builtin_amdgcn_raw_buffer_load_b64(rsrc, voff, soff, 0);
+      a.x += vr;
+      a.y += vi;
+      __builtin_amdgcn_raw_buffer_store_b64(a, rsrc, voff, soff, 0);
+    } else {
+      __builtin_amdgcn_raw_ptr_buffer_atomic_add_i32(vr, rsrc, voff, soff, 0);
+      __builtin_amdgcn_raw_ptr_buffer_atomic_add_i32(vi, rsrc, voff + 4, soff, 0);
+    }
+  }
+}
+
+// Flush one block of code-product accumulators (re = RE' - 2^24, im = IM'
+// - 2^24, exact integers), adding back 2^24 and the offset corrections:
+// rc[j] = (row RE, row IM) of register j's row, cc = (column RE, column
+// IM) of the lane's column (zeros for a piece that does not start the
+// chunk).  Same layout and modes as flush_block_f32.
+__device__ __forceinline__ void flush_block_f32_u4(int32_t *blk, const v16f &re, const v16f &im,
+                                                   int U, int mode, int lane,
+                                                   const v2i (&rc)[16], const v2i cc) {
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc(blk, (short)0, 2048 * 4, 0x00020000);
+  const int voff = ((lane >> 5) * 4 * 32 + (lane & 31)) * 8;
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    const int32_t vr = (int32_t)re[j] + (1 << 24) + rc[j].x + cc.x;
+    const int32_t m = (int32_t)im[j] + (1 << 24) + rc[j].y + cc.y;
+    const int32_t vi = U ? -m : m;
     const int soff = (8 * (j >> 2) + (j & 3)) * 32 * 8;
     if (mode == 0) {
       v2i v = {vr, vi};
@@ -325,7 +494,14 @@ __device__ __forceinline__ void tile_coords_small(int tb, int ntiles, int &ti, i
 // Row operands (4 groups x 2 planes per 128-row block) go through the LDS
 // ring by buffer->LDS DMA; each wave loads its own column operands (2
 // groups x 2 planes) straight into VGPRs, three stages ahead.
-template <bool PAIR, bool SMALL = false>
+//      U4: offset-binary code planes (the plan path): -Br is the
+//          complement 15 - u of every code, the accumulators start at
+//          -2^24 and hold code products (at most 450 per sample, so an item
+//          of at most kMaxStagesU4 stages ends below 2^24: every partial sum
+//          is an integer of magnitude <= 2^24, exact), and the flush adds
+//          2^24 and the offset corrections J.corr (the piece that starts the
+//          chunk adds them, once).
+template <bool PAIR, bool SMALL = false, bool U4 = false>
 __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, const int ti_a,
                                              const int ti_b, const int ct, const int st_begin,
                                              const int nst, uint8_t (*lds)[kSlotBytes]) {
@@ -357,6 +533,7 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   constexpr int kP0n = PAIR ? 4 : 2, kP1n = PAIR ? 2 : 1;
   static_assert(kDma == kP0n + kP1n, "the waits below count kDma + 8 loads per stage");
   static_assert(!(PAIR && SMALL), "a small tile is one 128-row block");
+  static_assert(!(U4 && SMALL), "offset-binary codes are the plan path's, whole tiles only");
   constexpr int kG = SMALL ? 2 : 4;                  // row groups this wave computes
   const int gr0 = SMALL ? 2 * (wave >> 1) : 0;       // the first of them
   const int rb = PAIR ? wave >> 1 : 0;               // this wave's row block
@@ -429,7 +606,8 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   };
 
   v16f re[kG][2], im[kG][2];
-  const v16f zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const float z0 = U4 ? -16777216.0f : 0.0f;  // U4: -2^24
+  const v16f zero = {z0, z0, z0, z0, z0, z0, z0, z0, z0, z0, z0, z0, z0, z0, z0, z0};
 #pragma unroll
   for (int i = 0; i < kG; i++)
 #pragma unroll
@@ -454,8 +632,11 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
     const v2i b = *(const v2i *)(lds0 + bA1[plane][k] + slot * kSlotBytes);
     return (v8i){a.x, a.y, a.z, a.w, b.x, b.y, 0, 0};
   };
-  const v8i sign = {0x20820820, 0x08208208, (int)0x82082082u, 0x20820820, 0x08208208,
-                    (int)0x82082082u, 0, 0};
+  // the sign bit of every 6-bit code, or (U4) its four value bits
+  const v8i sign = U4 ? (v8i){(int)0xCF3CF3CFu, (int)0xF3CF3CF3u, 0x3CF3CF3C, (int)0xCF3CF3CFu,
+                              (int)0xF3CF3CF3u, 0x3CF3CF3C, 0, 0}
+                      : (v8i){0x20820820, 0x08208208, (int)0x82082082u, 0x20820820, 0x08208208,
+                              (int)0x82082082u, 0, 0};
   // Row operands: two sets, alternating by stage parity: a stage computes
   // with one and reads the next stage's into the other.
   v8i AR[2][kG], AI[2][kG];
@@ -558,16 +739,55 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
       __builtin_amdgcn_sched_barrier(0);
     }
   }
+  if constexpr (U4) {
+    const bool first = st_begin == 0;
+    const v2i *rowc = (const v2i *)J.corr + (int64_t)f * Np;
+    const v2i *colc = (const v2i *)J.corr + ((int64_t)F + f) * Np;
+    v2i cc[2];
 #pragma unroll
-  for (int i = 0; i < kG; i++)
+    for (int j = 0; j < 2; j++)
+      cc[j] = first ? colc[(bx0 + j) * 32 + (lane & 31)] : (v2i){0, 0};
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-      const int by = by0 + i, bx = bx0 + j;
-      if (bx >= by && bx < nb) {
-        const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
-        flush_block_f32(vis + ((int64_t)f * NB + b) * 2048, re[i][j], im[i][j], U, mode, lane);
+    for (int i = 0; i < kG; i++) {
+      const int by = by0 + i;
+      // rows of register j: 32 by + 8 (j/4) + 4 (lane/32) + j%4
+      v2i rc[16];
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        v4i lo = {0, 0, 0, 0}, hi = {0, 0, 0, 0};
+        if (first) {
+          const v2i *p = rowc + by * 32 + 8 * g + 4 * (lane >> 5);
+          lo = *(const v4i *)p;
+          hi = *(const v4i *)(p + 2);
+        }
+        rc[4 * g] = (v2i){lo.x, lo.y};
+        rc[4 * g + 1] = (v2i){lo.z, lo.w};
+        rc[4 * g + 2] = (v2i){hi.x, hi.y};
+        rc[4 * g + 3] = (v2i){hi.z, hi.w};
+      }
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const int bx = bx0 + j;
+        if (bx >= by && bx < nb) {
+          const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
+          flush_block_f32_u4(vis + ((int64_t)f * NB + b) * 2048, re[i][j], im[i][j], U, mode,
+                             lane, rc, cc[j]);
+        }
       }
     }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kG; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const int by = by0 + i, bx = bx0 + j;
+        if (bx >= by && bx < nb) {
+          const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
+          flush_block_f32(vis + ((int64_t)f * NB + b) * 2048, re[i][j], im[i][j], U, mode,
+                          lane);
+        }
+      }
+  }
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
@@ -598,9 +818,9 @@ __global__ __launch_bounds__(256, 1) void corr_f6_kernel(GemmJob J, int head, in
   int ti_a, ti_b, ct;
   tile_coords((int)(item % per_f), J.ntiles, ti_a, ti_b, ct);
   if (ti_b >= 0 && ti_b != ti_a)
-    gemm_tile_f6<true>(J, (int)(item / per_f), ti_a, ti_b, ct, s0, ns, lds);
+    gemm_tile_f6<true, false, true>(J, (int)(item / per_f), ti_a, ti_b, ct, s0, ns, lds);
   else
-    gemm_tile_f6<false>(J, (int)(item / per_f), ti_a, ti_a, ct, s0, ns, lds);
+    gemm_tile_f6<false, false, true>(J, (int)(item / per_f), ti_a, ti_a, ct, s0, ns, lds);
 }
 
 // Batch: persistent workgroups take work items from 8 per-XCD queues (the
@@ -678,7 +898,18 @@ static int64_t f6_chunk_samples(const chx_plan *p) {
   int64_t tc = kWorkspaceBytes / per_sample;
   const int64_t desc = 0x7FFFFFFFLL / (Np / 2);  // KB * 2 * Np * 16 = tc * Np / 2 bytes
   if (tc > desc) tc = desc;
-  if (tc > (int64_t)kK * 32767) tc = (int64_t)kK * 32767;  // unpack grid.y = 2 KB <= 65535
+  if (tc > (int64_t)kK * 32767) tc = (int64_t)kK * 32767;  // unpack runs <= 2 KB <= 65535
+  // An array with at least a round of items (one per CU) takes chunks of at
+  // most kMaxStagesU4 stages, so that whole items cover a chunk (plain
+  // stores); a longer chunk would cut every item into atomic pieces.
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->cfg.device) !=
+          hipSuccess ||
+      cus < 1)
+    cus = 1;
+  if ((int64_t)p->cfg.num_freq * paired_tiles(p->ntiles) >= cus &&
+      tc > (int64_t)kK * kMaxStagesU4)
+    tc = (int64_t)kK * kMaxStagesU4;
   tc = tc / unit * unit;
   const int64_t T = (p->cfg.time_steps + unit - 1) / unit * unit;
   if (tc > T) tc = T;
@@ -686,9 +917,21 @@ static int64_t f6_chunk_samples(const chx_plan *p) {
   return tc;
 }
 
+// The workspace of one chunk: P0 and P1 code planes, then the unpack's
+// code-sum partials [run][F][Np] u32, then the offset corrections
+// [2][F][Np] int32 pairs.
+static int64_t f6_plane_bytes(const chx_plan *p) {
+  return 2 * (int64_t)p->cfg.num_freq * p->ntiles * kTile * f6_chunk_samples(p) * 3 / 4;
+}
+static int64_t f6_partial_bytes(const chx_plan *p) {
+  const int64_t Np = (int64_t)p->ntiles * kTile;
+  return f6_u4_runs(f6_chunk_samples(p) / kK, p->cfg.num_freq, (int)Np) * p->cfg.num_freq *
+         Np * 4;
+}
+
 size_t fp6_workspace_bytes(const chx_plan *p) {
   const int64_t Np = (int64_t)p->ntiles * kTile;
-  return (size_t)(2 * (int64_t)p->cfg.num_freq * Np * f6_chunk_samples(p) * 3 / 4);
+  return (size_t)(f6_plane_bytes(p) + f6_partial_bytes(p) + 2 * (int64_t)p->cfg.num_freq * Np * 8);
 }
 
 int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode,
@@ -703,6 +946,13 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
   if (cus < 1) cus = 1;
   const int64_t tc = f6_chunk_samples(p);
   const int64_t row_bytes = (int64_t)F * N;
+  // the unpack addresses the rows of a unit (at most 32 * 8) with 32-bit offsets
+  if (row_bytes > 0x7FFFFFFF / (32 * fp6_halves_per_unit(Np) + 1)) {
+    set_error_detail("FP6 matrix-core method: num_freq * num_elements too large for one unpack unit");
+    return CHX_ERR_SHAPE;
+  }
+  uint32_t *partials = (uint32_t *)(p->d_work + f6_plane_bytes(p));
+  int32_t *corr = (int32_t *)(p->d_work + f6_plane_bytes(p) + f6_partial_bytes(p));
   for (int64_t t0 = 0; t0 < T; t0 += tc) {
     const int64_t tv = T - t0 < tc ? T - t0 : tc;
     const int nst = (int)((tv + (int64_t)kK * kRing - 1) / ((int64_t)kK * kRing)) * kRing;
@@ -713,11 +963,11 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
     // take whole items (the chunk in one piece, plain stores); the items of
     // a last partial round are cut in time into the fewest pieces that
     // minimise its length and add atomically into zeroed output.  A chunk
-    // longer than kMaxStages cuts every item into pieces of at most
-    // kMaxStages stages (atomic).
+    // longer than kMaxStagesU4 cuts every item into pieces of at most
+    // kMaxStagesU4 stages (atomic).
     int64_t head = 0;  // items of the whole rounds
     int split = 1;
-    if (nst <= kMaxStages) head = items / cus * cus;
+    if (nst <= kMaxStagesU4) head = items / cus * cus;
     const int64_t tail = items - head;
     if (tail > 0) {
       double best = (double)((tail + cus - 1) / cus);
@@ -730,7 +980,7 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
       }
     }
     int sps = ((nst + split - 1) / split + kRing - 1) / kRing * kRing;
-    if (sps > kMaxStages) sps = kMaxStages;
+    if (sps > kMaxStagesU4) sps = kMaxStagesU4;
     split = (nst + sps - 1) / sps;
     const int kmode = (t0 == 0) ? mode : 1;
     const int tmode = split > 1 ? 2 : kmode;  // the tail's pieces add atomically
@@ -747,11 +997,13 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
       p->prof_ev->insert(p->prof_ev->end(), ev, ev + 3);
       CHX_HIP(hipEventRecord(ev[0], s));
     }
-    unpack_f6_kernel<<<dim3((unsigned)F, (unsigned)((Np + 1023) / 1024),
-                            (unsigned)((2 * KB + fp6_halves_per_unit(Np) - 1) /
-                                       fp6_halves_per_unit(Np))),
-                       dim3(256), 0, s>>>(in, p0, p1, N, F, Np, row_bytes, t0, tv, KB);
+    const int64_t nrun = f6_u4_runs(KB, F, Np);
+    unpack_f6_u4_kernel<<<dim3((unsigned)F, (unsigned)((Np + 1023) / 1024), (unsigned)nrun),
+                          dim3(256), 0, s>>>(in, p0, p1, partials, N, F, Np, row_bytes, t0, tv,
+                                             KB);
     CHX_HIP(hipGetLastError());
+    int rc1 = launch_u4_corrections(partials, corr, F, Np, (int)nrun, (int32_t)tv, s);
+    if (rc1) return rc1;
     if (p->prof) CHX_HIP(hipEventRecord(ev[1], s));
     GemmJob J;
     J.planes = p0;
@@ -767,7 +1019,7 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
     J.mode = kmode;
     J.stages_per_split = sps;
     J.small = 0;
-    J.corr = nullptr;
+    J.corr = corr;
     const int64_t grid = head + tail * split;
     if (grid > 0x7FFFFFFF) return CHX_ERR_SHAPE;
     corr_f6_kernel<<<dim3((unsigned)grid), dim3(256), 0, s>>>(J, (int)head, (int)tail, tmode);

@@ -66,6 +66,12 @@ constexpr int kMfmaRing = 4;     // stages in flight; stage counts are padded to
 size_t mfma_workspace_bytes(const chx_plan *p);
 int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis,
                      int mode, hipStream_t s);
+// Offset corrections of offset-binary code planes (both matrix-core plan
+// paths): partials [nrun][F][Np] of code sums SR | SI << 16 over a chunk of
+// n valid samples -> corr [2][F][Np] int32 pairs, rows (-8 (SR + SI) + 128 n,
+// -8 SR - 7 SI) then columns (-8 (SR + SI), 8 (SR - SI)).
+int launch_u4_corrections(const uint32_t *partials, int32_t *corr, int F, int Np, int nrun,
+                          int32_t n, hipStream_t s);
 
 // FP6 (E2M3) matrix-core path (chx_fp6.hip): same modes, bit-identical
 // results.

@@ -251,7 +251,9 @@ __global__ __launch_bounds__(256) void unpack_u4_kernel(
   // element) use an offset beyond every descriptor, so they read zeros
   // and their stores are dropped.  Streamed once: non-temporal (aux 2).
   const int rb = (int)row_bytes;  // 16 rows of a block fit in an int
-  const int voff_in = 4 * q < N ? f * N + 4 * q : 0x7FFFFFF0;
+  // lanes that read nothing: an offset past every descriptor (the row
+  // offsets added below keep it there, modulo 2^32)
+  const uint32_t voff_in = 4 * q < N ? (uint32_t)(f * N + 4 * q) : 0x80000000u;
   auto load = [&](uint32_t (&w)[16], int kb) {
     const int64_t t = 16 * (int64_t)kb;
     const int64_t rows = t_valid - t < 0 ? 0 : (t_valid - t > 16 ? 16 : t_valid - t);
@@ -259,7 +261,7 @@ __global__ __launch_bounds__(256) void unpack_u4_kernel(
         (void *)(in + (t0 + t) * row_bytes), (short)0, (int)(rows * row_bytes), 0x00020000);
 #pragma unroll
     for (int s = 0; s < 16; s++)
-      w[s] = __builtin_amdgcn_raw_buffer_load_b32(rs, voff_in + s * rb, 0, 2);
+      w[s] = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)(voff_in + (uint32_t)(s * rb)), 0, 2);
   };
   const int64_t plane_stride = (int64_t)F * KB * Np * 16;
   const int e_base = qb * 1024;
@@ -840,10 +842,8 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
                        dim3(256), 0, s>>>(in, p->d_work, partials, N, F, Np, row_bytes, t0, tv,
                                           KB);
     CHX_HIP(hipGetLastError());
-    const int64_t FN = (int64_t)F * Np;
-    u4_corrections_kernel<<<dim3((unsigned)((FN + 63) / 64)), dim3(256), 0, s>>>(
-        partials, corr, F, Np, (int)nrun, (int32_t)tv);
-    CHX_HIP(hipGetLastError());
+    int rc1 = launch_u4_corrections(partials, corr, F, Np, (int)nrun, (int32_t)tv, s);
+    if (rc1) return rc1;
     if (p->prof) CHX_HIP(hipEventRecord(ev[1], s));
     GemmJob J;
     J.planes = p->d_work;
@@ -867,6 +867,15 @@ int launch_corr_mfma(const chx_plan *p, const uint8_t *in, int32_t *vis, int mod
     CHX_HIP(hipGetLastError());
     if (p->prof) CHX_HIP(hipEventRecord(ev[2], s));
   }
+  return CHX_SUCCESS;
+}
+
+int launch_u4_corrections(const uint32_t *partials, int32_t *corr, int F, int Np, int nrun,
+                          int32_t n, hipStream_t s) {
+  const int64_t FN = (int64_t)F * Np;
+  u4_corrections_kernel<<<dim3((unsigned)((FN + 63) / 64)), dim3(256), 0, s>>>(partials, corr, F,
+                                                                             Np, nrun, n);
+  CHX_HIP(hipGetLastError());
   return CHX_SUCCESS;
 }
 
