@@ -10,8 +10,9 @@
 //    one persistent GEMM launch over work items (job, channel, 128x256 tile
 //    -- or, FP6, 128x128 for arrays up to small_tile_elements -- and stage
 //    range); a job's tiles are cut in time when max_item_stages asks
-//    or the chunk exceeds the item cap (kMaxItemStages), in which case their
-//    partial sums add atomically;
+//    or the chunk exceeds the item cap (format_max_stages), in which case
+//    their partial sums add atomically; FP6 rounds also launch the jobs'
+//    offset corrections (code sums from the unpack) between the two;
 //  * items are grouped per (job, channel) and the groups spread over 8
 //    per-XCD queues by longest-processing-time first, so a channel's tiles
 //    share one XCD's L2; inside a queue, long items run first.
@@ -36,6 +37,8 @@ struct chx_batch {
   struct Round {
     int ujob0 = 0, nujobs = 0;
     int64_t units = 0;
+    int cjob0 = 0, ncjobs = 0;  // FP6: offset-correction jobs
+    int64_t cblocks = 0;
   };
   std::vector<Round> round;
   struct Fill {
@@ -48,6 +51,7 @@ struct chx_batch {
   chx::GemmJob *d_gjobs = nullptr;
   chx::GemmItem *d_items = nullptr;
   chx::UnpackJob *d_ujobs = nullptr;
+  chx::CorrJob *d_cjobs = nullptr;
   int *d_qbound = nullptr;    // [rounds][9]
   int *d_counters = nullptr;  // [rounds][8]
   hipStream_t cap = nullptr;
@@ -170,8 +174,17 @@ int check_launch_tables(int fmt, const std::vector<UnpackJob> &ujobs,
         u.t_valid > (int64_t)spb * u.KB || u.Np % kMfmaTile || u.N > u.Np || u.kb_dst0 < 0 ||
         u.kb_dst0 + u.KB > u.KB_dst)
       return CHX_ERR_HIP;
-    if (fmt == kPlanesFp6 && u.planes1 != u.planes + 64LL * u.F * u.KB_dst * u.Np)
-      return CHX_ERR_HIP;
+    if (fmt == kPlanesFp6) {
+      // code sums: runs of at most 16 units (16-bit sums), inside their
+      // region; the unit rows of one input unit fit 32-bit offsets
+      if (u.planes1 != u.planes + 64LL * u.F * u.KB_dst * u.Np || !u.partials || u.prow0 < 0 ||
+          u.runs < 1 || (int64_t)u.runs * 16 < f6_u4_units(u.KB, u.Np) ||
+          u.row_bytes * (32LL * fp6_halves_per_unit(u.Np) + 1) > 0x7FFFFFFFLL)
+        return CHX_ERR_HIP;
+      const int64_t rows = (int64_t)(u.prow0 + u.runs) * u.F * u.Np * 4;
+      if ((const uint8_t *)u.partials < base || (const uint8_t *)u.partials + rows > end)
+        return CHX_ERR_HIP;
+    }
   }
   for (const GemmJob &g : gjobs) {
     const int stages = fmt == kPlanesFp6 ? g.KB : g.KB / 4;
@@ -181,6 +194,10 @@ int check_launch_tables(int fmt, const std::vector<UnpackJob> &ujobs,
         format_kb(fmt, stages) != g.KB || g.ntiles * kMfmaTile != g.Np || region > 0x7FFFFFFFLL)
       return CHX_ERR_HIP;
     if (fmt == kPlanesFp6 && g.planes1 != g.planes + 64LL * g.F * g.KB * g.Np) return CHX_ERR_HIP;
+    if (fmt == kPlanesFp6 &&
+        (!g.corr || (const uint8_t *)g.corr < base ||
+         (const uint8_t *)g.corr + 16LL * g.F * g.Np > end))
+      return CHX_ERR_HIP;
     if (g.small && fmt != kPlanesFp6) return CHX_ERR_HIP;
   }
   for (const GemmItem &w : items) {
@@ -242,6 +259,7 @@ void free_batch(chx_batch *b) {
   if (b->d_gjobs) (void)hipFree(b->d_gjobs);
   if (b->d_items) (void)hipFree(b->d_items);
   if (b->d_ujobs) (void)hipFree(b->d_ujobs);
+  if (b->d_cjobs) (void)hipFree(b->d_cjobs);
   if (b->d_qbound) (void)hipFree(b->d_qbound);
   if (b->d_counters) (void)hipFree(b->d_counters);
   delete b;
@@ -258,16 +276,23 @@ hipError_t upload(T **dst, const std::vector<T> &v) {
 // Host tables of one batch (uploaded once at creation).
 struct Tables {
   std::vector<UnpackJob> ujobs;
+  std::vector<CorrJob> cjobs;
   std::vector<GemmJob> gjobs;
   std::vector<GemmItem> items;
   std::vector<int> qbound;
 };
 
+// Workspace offsets of one job: its planes, and (FP6) its code-sum partial
+// rows and offset corrections.
+struct JobOffsets {
+  int64_t planes = 0, partials = 0, corr = 0;
+};
+
 // Chunk length, round count, per-job workspace offsets and total size.
-int plan_sizes(chx_batch *b, std::vector<int64_t> &off) {
+int plan_sizes(chx_batch *b, std::vector<JobOffsets> &off) {
   const int nj = (int)b->jobs.size();
   std::vector<int64_t> Np(nj);
-  off.assign(nj, 0);
+  off.assign(nj, JobOffsets{});
   b->cmacs = 0;
   const int64_t unit = (int64_t)kMfmaStage * format_ring(b->fmt);  // samples per ring
   int64_t per_unit = 0, maxT = 0;  // plane bytes of one ring of all jobs
@@ -294,9 +319,21 @@ int plan_sizes(chx_batch *b, std::vector<int64_t> &off) {
   b->rounds = (int)((maxT + chunk - 1) / chunk);
   int64_t ws = 0;
   for (int j = 0; j < nj; j++) {
-    off[j] = ws;
+    off[j].planes = ws;
     const int64_t stages = std::min(chunk, round_up(b->jobs[j].time_steps, unit)) / kMfmaStage;
     ws += format_plane_bytes(b->fmt, b->jobs[j].num_freq, Np[j], stages);
+  }
+  if (b->fmt == kPlanesFp6) {
+    for (int j = 0; j < nj; j++) {
+      const int64_t F = b->jobs[j].num_freq;
+      const int64_t stages = std::min(chunk, round_up(b->jobs[j].time_steps, unit)) / kMfmaStage;
+      off[j].partials = ws;
+      ws += round_up((int64_t)f6_u4_job_runs(format_kb(b->fmt, (int)stages), (int)Np[j]) * F *
+                         Np[j] * 4,
+                     256);
+      off[j].corr = ws;
+      ws += round_up(2 * F * Np[j] * 8, 256);
+    }
   }
   b->work_bytes = (size_t)ws;
   return CHX_SUCCESS;
@@ -304,10 +341,11 @@ int plan_sizes(chx_batch *b, std::vector<int64_t> &off) {
 
 // Every round's unpack jobs, GEMM jobs, items and queues, with plane
 // pointers into the workspace at `base`.
-int plan_tables(chx_batch *b, uint8_t *base, const std::vector<int64_t> &off, Tables &t) {
+int plan_tables(chx_batch *b, uint8_t *base, const std::vector<JobOffsets> &off, Tables &t) {
   const int nj = (int)b->jobs.size();
   const int64_t chunk = b->chunk;
   std::vector<UnpackJob> &ujobs = t.ujobs;
+  std::vector<CorrJob> &cjobs = t.cjobs;
   std::vector<GemmJob> &gjobs = t.gjobs;
   std::vector<GemmItem> &items = t.items;
   std::vector<int> &qbound = t.qbound;
@@ -318,6 +356,7 @@ int plan_tables(chx_batch *b, uint8_t *base, const std::vector<int64_t> &off, Ta
   for (int r = 0; r < b->rounds; r++) {
     chx_batch::Round R;
     R.ujob0 = (int)ujobs.size();
+    R.cjob0 = (int)cjobs.size();
     std::vector<ItemGroup> groups;
     for (int j = 0; j < nj; j++) {
       const chx_job_desc &d = b->jobs[j];
@@ -329,7 +368,10 @@ int plan_tables(chx_batch *b, uint8_t *base, const std::vector<int64_t> &off, Ta
       const int F = d.num_freq, N = d.num_elements;
       const int nt = (N + kMfmaTile - 1) / kMfmaTile;
       const int64_t Npj = round_up(N, kMfmaTile);
-      uint8_t *planes = base + off[j];
+      uint8_t *planes = base + off[j].planes;
+      const bool codes = b->fmt == kPlanesFp6;  // FP6: offset-binary codes and corrections
+      uint32_t *partials = codes ? (uint32_t *)(base + off[j].partials) : nullptr;
+      int32_t *corr = codes ? (int32_t *)(base + off[j].corr) : nullptr;
       uint8_t *planes1 = b->fmt == kPlanesFp6 ? planes + 64LL * F * KB * Npj : nullptr;
 
       UnpackJob u;
@@ -346,8 +388,23 @@ int plan_tables(chx_batch *b, uint8_t *base, const std::vector<int64_t> &off, Ta
       u.KB = KB;
       u.KB_dst = KB;
       u.kb_dst0 = 0;
+      u.partials = partials;
+      u.prow0 = 0;
+      u.runs = codes ? f6_u4_job_runs(KB, (int)Npj) : 0;
       ujobs.push_back(u);
       R.units += unpack_units(b->fmt, u);
+      if (codes) {
+        CorrJob c;
+        c.partials = partials;
+        c.corr = corr;
+        c.blk0 = R.cblocks;
+        c.F = F;
+        c.Np = (int)Npj;
+        c.nrun = u.runs;
+        c.n = (int)tv;
+        cjobs.push_back(c);
+        R.cblocks += ((int64_t)F * Npj + 63) / 64;
+      }
 
       // Items of this job: whole chunk per tile, or max_st-stage pieces;
       // 128 x 128 tiles for arrays of at most small_tile_elements (FP6).
@@ -369,11 +426,12 @@ int plan_tables(chx_batch *b, uint8_t *base, const std::vector<int64_t> &off, Ta
       g.mode = pieces > 1 ? 2 : (r == 0 ? 0 : 1);
       g.stages_per_split = per;
       g.small = small ? 1 : 0;
-      g.corr = nullptr;
+      g.corr = corr;
       if (pieces > 1 && r == 0) b->zero.push_back({d.d_vis, (int64_t)F * g.NB * 2048});
       gjobs.push_back(g);
     }
     R.nujobs = (int)ujobs.size() - R.ujob0;
+    R.ncjobs = (int)cjobs.size() - R.cjob0;
     spread_over_queues(groups, items, qbound);
     if (items.size() > 0x7FFFFFFFu) return CHX_ERR_SHAPE;
     b->round.push_back(R);
@@ -389,7 +447,7 @@ int check_tables(const chx_batch *b, const uint8_t *base, const Tables &t) {
 
 // Allocates the workspace, builds and checks the tables, uploads them.
 int plan_batch(chx_batch *b) {
-  std::vector<int64_t> off;
+  std::vector<JobOffsets> off;
   int rc = plan_sizes(b, off);
   if (rc) return rc;
   CHX_HIP(hipMalloc((void **)&b->d_work, b->work_bytes));
@@ -398,6 +456,7 @@ int plan_batch(chx_batch *b) {
   if (!rc) rc = check_tables(b, b->d_work, t);
   if (rc) return rc;
   hipError_t e = upload(&b->d_ujobs, t.ujobs);
+  if (e == hipSuccess) e = upload(&b->d_cjobs, t.cjobs);
   if (e == hipSuccess) e = upload(&b->d_gjobs, t.gjobs);
   if (e == hipSuccess) e = upload(&b->d_items, t.items);
   if (e == hipSuccess) e = upload(&b->d_qbound, t.qbound);
@@ -414,6 +473,8 @@ int enqueue(const chx_batch *b, hipStream_t s) {
   for (int r = 0; r < b->rounds && !rc; r++) {
     const chx_batch::Round &R = b->round[r];
     rc = launch_unpack_batch(b->fmt, b->d_ujobs + R.ujob0, R.nujobs, R.units, s);
+    if (!rc && R.ncjobs > 0)
+      rc = launch_u4_corrections_batch(b->d_cjobs + R.cjob0, R.ncjobs, R.cblocks, s);
     if (!rc)
       rc = launch_gemm_batch(b->fmt, b->d_gjobs, b->d_items, b->d_qbound + 9 * r,
                              b->d_counters + 8 * r, b->tune.workgroups, s);
@@ -499,7 +560,7 @@ int chx_batch_plan(const chx_job_desc *jobs, int32_t num_jobs, const chx_batch_t
   b.fmt = fmt;
   b.jobs.assign(jobs, jobs + num_jobs);
   b.tune = t;
-  std::vector<int64_t> off;
+  std::vector<JobOffsets> off;
   rc = plan_sizes(&b, off);
   if (rc) return rc;
   // Plane pointers relative to a stand-in base (never dereferenced).

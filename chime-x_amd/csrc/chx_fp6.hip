@@ -6,27 +6,20 @@
 // 32x32x32 one takes, and accumulates in FP32.  Per 32x32 block two
 // accumulators:
 //   RE = Ar.Br^T + Ai.Bi^T,  IM = Ar.Bi^T + Ai.(-Br)^T,  im = U ? -IM : IM.
-// Two encodings of a nibble u (value v = u - 8):
-//  * plan path: u/8, whose E2M3 code is u itself (u < 8: the subnormal u/8;
-//    u >= 8: exponent 1, mantissa u - 8); -Br is the complement 15 - u.  The
-//    accumulators hold code products (the same RE', IM' and offset
-//    corrections as the int8 plan path, chx_mfma.hip), at most 450 per
-//    sample: they start at -2^24 and a work item covers at most
-//    kMaxStagesU4 = 1024 stages of 64 samples, so every partial sum is an
-//    integer of magnitude <= 2^24, exact in FP32.  Codes switch four bits
-//    and every product is >= 0, so the power-limited matrix cores hold a
-//    higher clock than with signed values.
-//  * batch and serve: v/8, |v| = 0..7 the subnormals m/8 and |v| = 8 1.0, so
-//    every code is sign<<5 | |v|; -Br flips the sign bits.  Every product is
-//    an integer of magnitude <= 64, a sample adds at most 128, and a work
-//    item covers at most kMaxItemStages = 2048 stages: partial sums <= 2^24.
-// (The signed values as E3M2 integers, all normal numbers with scales 2^0,
-// run 5% slower: on random data the matrix cores are power-limited, and the
-// subnormal operands hold a higher clock.)  Either way the int32
-// visibilities are bit-identical to the int8 path's.
+// The nibble u (value v = u - 8) is carried as u/8, whose E2M3 code is u
+// itself (u < 8: the subnormal u/8; u >= 8: exponent 1, mantissa u - 8);
+// -Br is the complement 15 - u.  The accumulators hold code products (the
+// RE', IM' of the int8 plan path, chx_mfma.hip), at most 450 per sample:
+// they start at -2^24 and a work item covers at most kMaxItemStagesFp6 =
+// 1024 stages of 64 samples, so every partial sum is an integer of
+// magnitude <= 2^24, exact in FP32, and the flush adds 2^24 and the
+// per-element offset corrections.  Codes switch four bits and every product
+// is >= 0, so the power-limited matrix cores hold a higher clock than with
+// signed values (and subnormal E2M3 operands a higher one than normal E3M2
+// integers).  The int32 visibilities are bit-identical to the int8 path's.
 //
 // Two passes per time chunk:
-//  1. unpack_f6_u4_kernel / unpack_f6_batch_kernel (HBM-bound):
+//  1. unpack_f6_u4_kernel / unpack_f6_u4_batch_kernel (HBM-bound):
 //     [time][freq][element] bytes -> E2M3
 //     codes, bit-packed 32 per 24 bytes (code j at bits 6j..6j+5, little
 //     endian), as two arrays per plane: P0 = bytes 0-15 and P1 = bytes 16-23
@@ -63,34 +56,21 @@ typedef __attribute__((address_space(3))) void lds_void;
 constexpr int kTile = kMfmaTile;     // 128: tile rows; a tile has 2 * kTile columns
 constexpr int kK = 64;               // samples per k-step (= per stage)
 constexpr int kRing = kFp6Ring;      // LDS stages in the ring
-constexpr int kMaxStages = kMaxItemStages;  // stages of one work item
-// stages of one work item on offset-binary codes (the plan path): 450 * 64
-// * 1024 = 29491200 < 2^25, so from -2^24 every partial sum stays <= 2^24
-constexpr int kMaxStagesU4 = 1024;
+// stages of one work item: code products add at most 450 per sample, and
+// 450 * 64 * 1024 = 29491200 < 2^25, so from -2^24 every partial sum stays
+// an integer of magnitude <= 2^24
+constexpr int kMaxStages = kMaxItemStagesFp6;
 constexpr int kRowGroups = 4;        // 32-element row groups per stage (in LDS)
 constexpr int kP0Bytes = 2 * kRowGroups * 1024;  // [plane 2][group 4][64 lanes][16 B]
 constexpr int kP1Bytes = 2 * kRowGroups * 512;   // [plane 2][group 4][64 lanes][8 B]
 constexpr int kSlotBytes = kP0Bytes + kP1Bytes;  // one 128-row block of one stage
 constexpr int kRingBytes = kRing * kSlotBytes;   // a ring of one row block
-static_assert(kMaxStages % kRing == 0 && kMaxStagesU4 % kRing == 0,
-              "work items hold whole rings");
+static_assert(kMaxStages % kRing == 0, "work items hold whole rings");
 
 // MFMA operand format and block scale: E2M3 (format 2 of the f8f6f4
 // instruction) holding v/8, scaled back by the E8M0 exponent 127 + 3.
 constexpr int kFmtE2M3 = 2;
 constexpr int kScale2p3 = 127 + 3;
-
-// Four nibbles (one per byte, 0-15) -> the E2M3 codes of (nibble - 8) / 8
-// (one per byte), by a 16-entry byte table in four words: entries 0-7
-// (v = -8..-1: sign | 8..1) in the first pair, 8-15 (v = 0..7) in the
-// second.
-__device__ __forceinline__ uint32_t fp6_codes(uint32_t n) {
-  const uint32_t sel = n & 0x07070707u;
-  const uint32_t neg = __builtin_amdgcn_perm(0x21222324u, 0x25262728u, sel);
-  const uint32_t pos = __builtin_amdgcn_perm(0x07060504u, 0x03020100u, sel);
-  const uint32_t m = ((n >> 3) & 0x01010101u) * 0xFFu;  // 0xFF where nibble >= 8
-  return (pos & m) | (neg & ~m);
-}
 
 // Four 6-bit codes (one per byte) -> 24 contiguous bits.
 __device__ __forceinline__ uint32_t pack24(uint32_t c) {
@@ -122,155 +102,31 @@ __device__ __forceinline__ void transpose4(uint32_t r0, uint32_t r1, uint32_t r2
   c3 = __builtin_amdgcn_perm(t23h, t01h, 0x07060302u);
 }
 
-// Pass 1.  Workgroup = 256 element quads (1024 elements) x one 32-sample
-// half of a 64-sample block of one channel; an array of Np < 1024 elements
-// takes nh = floor(1024 / Np) consecutive halves instead, so all threads
-// work.  Thread: 32 word loads (one per sample row; 256 contiguous bytes per
-// wave and row), 8 byte transposes, then per plane: codes, bit packing, LDS
-// staging, and full-run global stores (1 KB of P0 or 512 B of P1 per wave
-// instruction; the halves of one unit are consecutive destination rows).
-// Source half hk (relative to t0) lands at destination half 2 kb_dst0 + hk
-// of a layout with KB_dst blocks; the unit covers halves hk0 .. hk0+nh-1.
-template <bool MULTI>  // MULTI: Np < 1024, several halves per unit
-__device__ __forceinline__ void unpack_f6_unit(const uint8_t *__restrict__ in,
-                                               uint8_t *__restrict__ p0,
-                                               uint8_t *__restrict__ p1, int N, int F, int Np,
-                                               int64_t row_bytes, int64_t t0, int64_t t_valid,
-                                               int KB_dst, int kb_dst0, int qb, int hk0, int nh,
-                                               int f, v4u *s0, v2u *s1) {
-  const int tid = threadIdx.x;
-  const int epq = MULTI ? Np / 4 : 256;   // element quads per half in this unit
-  const int sub = MULTI ? tid / epq : 0;  // which half of the unit
-  const bool active = !MULTI || sub < nh;
-  const int q = MULTI ? tid % epq : qb * 256 + tid;  // element quad
-  const bool evalid = active && 4 * q < N;
-  const int64_t ts = 32 * (int64_t)(hk0 + sub);  // first source sample of the half
-  uint32_t c[4][8];  // [element][4-sample group]
-  {
-    const uint8_t *src = in + (t0 + ts) * row_bytes + (int64_t)f * N + 4 * q;
-    uint32_t w[32];
-#pragma unroll
-    for (int s = 0; s < 32; s++) {
-      const bool ok = evalid && (ts + s < t_valid);
-      // streamed once: non-temporal loads (and stores below)
-      w[s] = ok ? __builtin_nontemporal_load((const uint32_t *)(src + s * row_bytes))
-                : 0x88888888u;  // 0x88 -> code 0
-    }
-#pragma unroll
-    for (int m = 0; m < 8; m++)
-      transpose4(w[4 * m], w[4 * m + 1], w[4 * m + 2], w[4 * m + 3], c[0][m], c[1][m], c[2][m],
-                 c[3][m]);
-  }
-  const int e_base = qb * 1024;
-  const int n_el = MULTI ? nh * Np : (Np - e_base < 1024 ? Np - e_base : 1024);
-  const int el0 = MULTI ? sub * Np + 4 * q : 4 * tid;  // this thread's entries
-#pragma unroll
-  for (int pl = 0; pl < 2; pl++) {
-    if (active) {
-#pragma unroll
-      for (int e = 0; e < 4; e++) {
-        uint32_t code[8];
-#pragma unroll
-        for (int m = 0; m < 8; m++)
-          code[m] = fp6_codes(pl == 0 ? (c[e][m] >> 4) & 0x0F0F0F0Fu : c[e][m] & 0x0F0F0F0Fu);
-        const int el = el0 + e;
-        const int slot = el + (el >> 4);  // one pad entry per 16
-        v4u a;
-        v2u b;
-        pack32(code, a, b);
-        s0[slot] = a;
-        s1[slot] = b;
-      }
-    }
-    __syncthreads();
-    // (plane, channel, destination half) of the unit's first half
-    const int64_t row = ((int64_t)pl * F + f) * 2 * KB_dst + 2 * (int64_t)kb_dst0 + hk0;
-    uint8_t *d0 = p0 + (row * Np + e_base) * 16;
-    uint8_t *d1 = p1 + (row * Np + e_base) * 8;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int el = k * 256 + tid;
-      if (el < n_el) {
-        const int slot = el + (el >> 4);
-        __builtin_nontemporal_store(s0[slot], (v4u *)(d0 + el * 16));
-        __builtin_nontemporal_store(s1[slot], (v2u *)(d1 + el * 8));
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// Batch: one launch unpacks the chunks of many jobs; workgroup g belongs to
-// the job j with jobs[j].unit0 <= g < jobs[j+1].unit0 (binary search); its
-// unit within the job is (unit of halves, element group, channel), channel
-// fastest.
-__global__ __launch_bounds__(256) void unpack_f6_batch_kernel(const UnpackJob *__restrict__ jobs,
-                                                              int njobs) {
-  __shared__ __attribute__((aligned(16))) v4u s0[1024 + 64];
-  __shared__ __attribute__((aligned(16))) v2u s1[1024 + 64];
-  const int64_t g = blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
-  int lo = 0, hi = njobs - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) / 2;
-    if (jobs[mid].unit0 <= g) lo = mid; else hi = mid - 1;
-  }
-  const UnpackJob &J = jobs[lo];
-  const int64_t u = g - J.unit0;
-  const int nq = (J.Np + 1023) / 1024;
-  const int hpu = fp6_halves_per_unit(J.Np);
-  const int nhu = (2 * J.KB + hpu - 1) / hpu;  // units of halves per element group
-  if (u >= (int64_t)nq * nhu * J.F) return;  // grid padding (uniform per workgroup)
-  const int f = (int)(u % J.F);  // channel fastest: neighbours read whole rows
-  const int qb = (int)((u / J.F) % nq);
-  const int hk0 = (int)(u / ((int64_t)J.F * nq)) * hpu;  // first half within the job
-  const int nh = 2 * J.KB - hk0 < hpu ? 2 * J.KB - hk0 : hpu;
-  if (J.Np >= 1024)
-    unpack_f6_unit<false>(J.in, J.planes, J.planes1, J.N, J.F, J.Np, J.row_bytes, J.t0,
-                          J.t_valid, J.KB_dst, J.kb_dst0, qb, hk0, 1, f, s0, s1);
-  else
-    unpack_f6_unit<true>(J.in, J.planes, J.planes1, J.N, J.F, J.Np, J.row_bytes, J.t0,
-                         J.t_valid, J.KB_dst, J.kb_dst0, 0, hk0, nh, f, s0, s1);
-}
-
-// Pass 1 of the plan path: offset-binary codes.  The nibble u (value u - 8)
-// is carried as u/8, whose E2M3 code is u itself (u < 8: the subnormal u/8;
-// u >= 8: exponent 1, mantissa u - 8), so this unpack only transposes and
-// packs the nibbles.  A unit is 1024 elements x one 32-sample half, or for
-// an array under 1024 elements fp6_halves_per_unit(Np) consecutive halves of
-// all its elements (as unpack_f6_unit).  Grid x = channel, y = element
-// group, z = run: workgroup `run` takes the units run, run + R, ... of the
-// chunk (R = gridDim.z, about kF6U4Workgroups workgroups in all), so that
-// the resident workgroups work on neighbouring units at any time.  Loads and
-// stores go through buffer descriptors (branch-free: lanes past N or past
-// the valid samples read zeros, stores past the unit are dropped),
+// Pass 1: offset-binary codes.  The nibble u (value u - 8) is carried as
+// u/8, whose E2M3 code is u itself (u < 8: the subnormal u/8; u >= 8:
+// exponent 1, mantissa u - 8), so the unpack only transposes and packs the
+// nibbles.  A unit is 1024 elements x one 32-sample half, or for an array
+// under 1024 elements fp6_halves_per_unit(Np) consecutive halves of all its
+// elements.  One workgroup = one run of channel f and element group qb: the
+// units run, run + R, ... of the chunk, so that the resident workgroups
+// work on neighbouring units at any time.  Per unit: 32 word loads per
+// thread (one per sample row; 256 contiguous bytes per wave and row), 8
+// byte transposes, then per plane: codes, bit packing, LDS staging and
+// full-run stores (1 KB of P0 or 512 B of P1 per wave instruction; the
+// halves of one unit are consecutive destination rows).  Loads and stores
+// go through buffer descriptors (branch-free: lanes past N or past the
+// valid samples read zeros, stores past the unit are dropped),
 // non-temporal; the workgroups resident on a CU overlap each other's loads
-// and stores.
-// Besides the planes each workgroup writes its elements' code sums over its
-// units, SR | SI << 16 (at most 15 * 32 * 136 = 65280 with at most 136
-// halves per element and workgroup), into partials[run][f][element].
-// Samples past t_valid and elements past N are code 0.
-constexpr int kF6U4Workgroups = 1024;
-constexpr int kF6U4MaxHalves = 136;
-__host__ __device__ inline int64_t f6_u4_units(int64_t KB, int Np) {
-  const int hpu = fp6_halves_per_unit(Np);
-  return (2 * KB + hpu - 1) / hpu;
-}
-__host__ __device__ inline int64_t f6_u4_runs(int64_t KB, int64_t F, int Np) {
-  const int64_t units = f6_u4_units(KB, Np);
-  const int64_t per = F * ((Np + 1023) / 1024);
-  int64_t r = (kF6U4Workgroups + per - 1) / per;
-  const int64_t upw = kF6U4MaxHalves / fp6_halves_per_unit(Np);  // units per workgroup
-  const int64_t need = (units + upw - 1) / upw;
-  if (r < need) r = need;
-  return r < units ? r : units;
-}
-__global__ __launch_bounds__(256) void unpack_f6_u4_kernel(
+// and stores.  Source half hk (relative to t0) lands at destination half 2
+// kb_dst0 + hk of a layout with KB_dst blocks.  Besides the planes the run
+// writes its elements' code sums over its units, SR | SI << 16 (at most
+// 15 * 32 * 136 = 65280 with at most 136 halves per element and run), into
+// prow[f][element].  Samples past t_valid and elements past N are code 0.
+__device__ __forceinline__ void unpack_f6_u4_run(
     const uint8_t *__restrict__ in, uint8_t *__restrict__ p0, uint8_t *__restrict__ p1,
-    uint32_t *__restrict__ partials, int N, int F, int Np, int64_t row_bytes, int64_t t0,
-    int64_t t_valid, int KB) {
-  __shared__ __attribute__((aligned(16))) v4u s0[1024 + 64];  // [element] P0, padded
-  __shared__ __attribute__((aligned(16))) v2u s1[1024 + 64];  // [element] P1, padded
-  const int f = blockIdx.x, qb = blockIdx.y, run = blockIdx.z, R = gridDim.z;
+    uint32_t *__restrict__ prow, int N, int F, int Np, int64_t row_bytes, int64_t t0,
+    int64_t t_valid, int KB, int KB_dst, int kb_dst0, int f, int qb, int run, int R,
+    v4u *s0, v2u *s1) {
   const int tid = threadIdx.x;
   const bool multi = Np < 1024;
   const int hpu = fp6_halves_per_unit(Np);
@@ -329,19 +185,17 @@ __global__ __launch_bounds__(256) void unpack_f6_u4_kernel(
           sr[e] = __builtin_amdgcn_sad_u8(bs, 0u, sr[e]);
         else
           si[e] = __builtin_amdgcn_sad_u8(bs, 0u, si[e]);
-        const int el = el0 + e;
+        const int el = el0 + e;  // = 4 tid + e < 1024
         const int slot = el + (el >> 4);  // one pad entry per 16
         v4u a;
         v2u b;
         pack32(code, a, b);
-        if (el < 1024) {
-          s0[slot] = a;
-          s1[slot] = b;
-        }
+        s0[slot] = a;
+        s1[slot] = b;
       }
       __syncthreads();
       // (plane, channel, destination half) of the unit's first half
-      const int64_t row = ((int64_t)pl * F + f) * 2 * KB + hk0;
+      const int64_t row = ((int64_t)pl * F + f) * 2 * KB_dst + 2 * (int64_t)kb_dst0 + hk0;
       const __amdgpu_buffer_rsrc_t d0 = __builtin_amdgcn_make_buffer_rsrc(
           p0 + (row * Np + e_base) * 16, (short)0, n_el * 16, 0x00020000);
       const __amdgpu_buffer_rsrc_t d1 = __builtin_amdgcn_make_buffer_rsrc(
@@ -356,7 +210,7 @@ __global__ __launch_bounds__(256) void unpack_f6_u4_kernel(
       __syncthreads();
     }
   }
-  uint32_t *dst = partials + ((int64_t)run * F + f) * Np;
+  uint32_t *dst = prow + (int64_t)f * Np;
   if (!multi) {
     if (4 * q < Np)
       *(v4u *)(dst + 4 * q) = (v4u){sr[0] | si[0] << 16, sr[1] | si[1] << 16,
@@ -377,41 +231,65 @@ __global__ __launch_bounds__(256) void unpack_f6_u4_kernel(
   }
 }
 
-// Flush one block's FP32 accumulators (exact integers) into int32
-// visibilities.  Lane l owns column n = l%32 and rows m = 8*(j/4) +
-// 4*(l/32) + j%4 of register j.  re holds Ar.Br + Ai.Bi, im holds Ar.Bi -
-// Ai.Br.  mode 0 assigns, 1 adds, 2 adds atomically.
-__device__ __forceinline__ void flush_block_f32(int32_t *blk, const v16f &re, const v16f &im,
-                                                int U, int mode, int lane) {
-  const __amdgpu_buffer_rsrc_t rsrc =
-      __builtin_amdgcn_make_buffer_rsrc(blk, (short)0, 2048 * 4, 0x00020000);
-  const int voff = ((lane >> 5) * 4 * 32 + (lane & 31)) * 8;
-#pragma unroll
-  for (int j = 0; j < 16; j++) {
-    const int32_t vr = (int32_t)re[j];  // an integer of magnitude <= 2^24: exact
-    const int32_t vi = (int32_t)(U ? -im[j] : im[j]);
-    const int soff = (8 * (j >> 2) + (j & 3)) * 32 * 8;
-    if (mode == 0) {
-      v2i v = {vr, vi};
-      __builtin_amdgcn_raw_buffer_store_b64(v, rsrc, voff, soff, 0);
-    } else if (mode == 1) {
-      v2i a = __builtin_amdgcn_raw_buffer_load_b64(rsrc, voff, soff, 0);
-      a.x += vr;
-      a.y += vi;
-      __builtin_amdgcn_raw_buffer_store_b64(a, rsrc, voff, soff, 0);
-    } else {
-      __builtin_amdgcn_raw_ptr_buffer_atomic_add_i32(vr, rsrc, voff, soff, 0);
-      __builtin_amdgcn_raw_ptr_buffer_atomic_add_i32(vi, rsrc, voff + 4, soff, 0);
-    }
+// Plan chunk: grid x = channel, y = element group, z = run (about
+// kF6U4Workgroups workgroups in all); run r writes partials row r.
+constexpr int kF6U4Workgroups = 1024;
+constexpr int kF6U4MaxHalves = 136;
+__host__ __device__ inline int64_t f6_u4_runs(int64_t KB, int64_t F, int Np) {
+  const int64_t units = f6_u4_units(KB, Np);
+  const int64_t per = F * ((Np + 1023) / 1024);
+  int64_t r = (kF6U4Workgroups + per - 1) / per;
+  const int64_t upw = kF6U4MaxHalves / fp6_halves_per_unit(Np);  // units per workgroup
+  const int64_t need = (units + upw - 1) / upw;
+  if (r < need) r = need;
+  return r < units ? r : units;
+}
+__global__ __launch_bounds__(256, 4) void unpack_f6_u4_kernel(
+    const uint8_t *__restrict__ in, uint8_t *__restrict__ p0, uint8_t *__restrict__ p1,
+    uint32_t *__restrict__ partials, int N, int F, int Np, int64_t row_bytes, int64_t t0,
+    int64_t t_valid, int KB) {
+  __shared__ __attribute__((aligned(16))) v4u s0[1024 + 64];  // [element] P0, padded
+  __shared__ __attribute__((aligned(16))) v2u s1[1024 + 64];  // [element] P1, padded
+  unpack_f6_u4_run(in, p0, p1, partials + (int64_t)blockIdx.z * F * Np, N, F, Np, row_bytes,
+                   t0, t_valid, KB, KB, 0, blockIdx.x, blockIdx.y, blockIdx.z, gridDim.z, s0,
+                   s1);
+}
+
+// Batch: one launch unpacks the chunks of many jobs; workgroup g belongs to
+// the job j with jobs[j].unit0 <= g < jobs[j+1].unit0 (binary search); its
+// index within the job is (run, element group, channel), channel fastest:
+// neighbouring workgroups read the same input rows.
+__global__ __launch_bounds__(256, 4) void unpack_f6_u4_batch_kernel(
+    const UnpackJob *__restrict__ jobs, int njobs) {
+  __shared__ __attribute__((aligned(16))) v4u s0[1024 + 64];
+  __shared__ __attribute__((aligned(16))) v2u s1[1024 + 64];
+  const int64_t g = blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].unit0 <= g) lo = mid; else hi = mid - 1;
   }
+  const UnpackJob &J = jobs[lo];
+  const int64_t u = g - J.unit0;
+  const int nq = (J.Np + 1023) / 1024;
+  if (u >= (int64_t)nq * J.runs * J.F) return;  // grid padding (uniform per workgroup)
+  const int f = (int)(u % J.F);
+  const int qb = (int)((u / J.F) % nq);
+  const int run = (int)(u / ((int64_t)J.F * nq));
+  unpack_f6_u4_run(J.in, J.planes, J.planes1,
+                   J.partials + (int64_t)(J.prow0 + run) * J.F * J.Np, J.N, J.F, J.Np,
+                   J.row_bytes, J.t0, J.t_valid, J.KB, J.KB_dst, J.kb_dst0, f, qb, run, J.runs,
+                   s0, s1);
 }
 
 // Flush one block of code-product accumulators (re = RE' - 2^24, im = IM'
-// - 2^24, exact integers), adding back 2^24 and the offset corrections:
-// rc[j] = (row RE, row IM) of register j's row, cc = (column RE, column
-// IM) of the lane's column (zeros for a piece that does not start the
-// chunk).  Same layout and modes as flush_block_f32.
-__device__ __forceinline__ void flush_block_f32_u4(int32_t *blk, const v16f &re, const v16f &im,
+// - 2^24, exact integers) into int32 visibilities, adding back 2^24 and the
+// offset corrections: rc[j] = (row RE, row IM) of register j's row, cc =
+// (column RE, column IM) of the lane's column (zeros for a piece that does
+// not start the chunk).  Lane l owns column n = l%32 and rows m = 8*(j/4)
+// + 4*(l/32) + j%4 of register j.  mode 0 assigns, 1 adds, 2 adds
+// atomically.
+__device__ __forceinline__ void flush_block_f32(int32_t *blk, const v16f &re, const v16f &im,
                                                    int U, int mode, int lane,
                                                    const v2i (&rc)[16], const v2i cc) {
   const __amdgpu_buffer_rsrc_t rsrc =
@@ -494,14 +372,13 @@ __device__ __forceinline__ void tile_coords_small(int tb, int ntiles, int &ti, i
 // Row operands (4 groups x 2 planes per 128-row block) go through the LDS
 // ring by buffer->LDS DMA; each wave loads its own column operands (2
 // groups x 2 planes) straight into VGPRs, three stages ahead.
-//      U4: offset-binary code planes (the plan path): -Br is the
-//          complement 15 - u of every code, the accumulators start at
-//          -2^24 and hold code products (at most 450 per sample, so an item
-//          of at most kMaxStagesU4 stages ends below 2^24: every partial sum
-//          is an integer of magnitude <= 2^24, exact), and the flush adds
-//          2^24 and the offset corrections J.corr (the piece that starts the
-//          chunk adds them, once).
-template <bool PAIR, bool SMALL = false, bool U4 = false>
+// Operands are offset-binary codes: -Br is the complement 15 - u of every
+// code, the accumulators start at -2^24 and hold code products (at most 450
+// per sample, so an item of at most kMaxStages stages ends below 2^24:
+// every partial sum is an integer of magnitude <= 2^24, exact), and the
+// flush adds 2^24 and the offset corrections J.corr (the piece that starts
+// the chunk adds them, once).
+template <bool PAIR, bool SMALL = false>
 __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, const int ti_a,
                                              const int ti_b, const int ct, const int st_begin,
                                              const int nst, uint8_t (*lds)[kSlotBytes]) {
@@ -533,7 +410,6 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   constexpr int kP0n = PAIR ? 4 : 2, kP1n = PAIR ? 2 : 1;
   static_assert(kDma == kP0n + kP1n, "the waits below count kDma + 8 loads per stage");
   static_assert(!(PAIR && SMALL), "a small tile is one 128-row block");
-  static_assert(!(U4 && SMALL), "offset-binary codes are the plan path's, whole tiles only");
   constexpr int kG = SMALL ? 2 : 4;                  // row groups this wave computes
   const int gr0 = SMALL ? 2 * (wave >> 1) : 0;       // the first of them
   const int rb = PAIR ? wave >> 1 : 0;               // this wave's row block
@@ -606,7 +482,7 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   };
 
   v16f re[kG][2], im[kG][2];
-  const float z0 = U4 ? -16777216.0f : 0.0f;  // U4: -2^24
+  const float z0 = -16777216.0f;  // -2^24
   const v16f zero = {z0, z0, z0, z0, z0, z0, z0, z0, z0, z0, z0, z0, z0, z0, z0, z0};
 #pragma unroll
   for (int i = 0; i < kG; i++)
@@ -632,11 +508,9 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
     const v2i b = *(const v2i *)(lds0 + bA1[plane][k] + slot * kSlotBytes);
     return (v8i){a.x, a.y, a.z, a.w, b.x, b.y, 0, 0};
   };
-  // the sign bit of every 6-bit code, or (U4) its four value bits
-  const v8i sign = U4 ? (v8i){(int)0xCF3CF3CFu, (int)0xF3CF3CF3u, 0x3CF3CF3C, (int)0xCF3CF3CFu,
-                              (int)0xF3CF3CF3u, 0x3CF3CF3C, 0, 0}
-                      : (v8i){0x20820820, 0x08208208, (int)0x82082082u, 0x20820820, 0x08208208,
-                              (int)0x82082082u, 0, 0};
+  // the four value bits of every 6-bit code: u -> 15 - u
+  const v8i sign = {(int)0xCF3CF3CFu, (int)0xF3CF3CF3u, 0x3CF3CF3C, (int)0xCF3CF3CFu,
+                    (int)0xF3CF3CF3u, 0x3CF3CF3C, 0, 0};
   // Row operands: two sets, alternating by stage parity: a stage computes
   // with one and reads the next stage's into the other.
   v8i AR[2][kG], AI[2][kG];
@@ -739,54 +613,41 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
       __builtin_amdgcn_sched_barrier(0);
     }
   }
-  if constexpr (U4) {
-    const bool first = st_begin == 0;
-    const v2i *rowc = (const v2i *)J.corr + (int64_t)f * Np;
-    const v2i *colc = (const v2i *)J.corr + ((int64_t)F + f) * Np;
-    v2i cc[2];
+  // Corrections: the piece that starts the chunk adds them, once.
+  const bool first = st_begin == 0;
+  const v2i *rowc = (const v2i *)J.corr + (int64_t)f * Np;
+  const v2i *colc = (const v2i *)J.corr + ((int64_t)F + f) * Np;
+  v2i cc[2];
 #pragma unroll
-    for (int j = 0; j < 2; j++)
-      cc[j] = first ? colc[(bx0 + j) * 32 + (lane & 31)] : (v2i){0, 0};
+  for (int j = 0; j < 2; j++)
+    cc[j] = first ? colc[(bx0 + j) * 32 + (lane & 31)] : (v2i){0, 0};
 #pragma unroll
-    for (int i = 0; i < kG; i++) {
-      const int by = by0 + i;
-      // rows of register j: 32 by + 8 (j/4) + 4 (lane/32) + j%4
-      v2i rc[16];
+  for (int i = 0; i < kG; i++) {
+    const int by = by0 + i;
+    // rows of register j: 32 by + 8 (j/4) + 4 (lane/32) + j%4
+    v2i rc[16];
 #pragma unroll
-      for (int g = 0; g < 4; g++) {
-        v4i lo = {0, 0, 0, 0}, hi = {0, 0, 0, 0};
-        if (first) {
-          const v2i *p = rowc + by * 32 + 8 * g + 4 * (lane >> 5);
-          lo = *(const v4i *)p;
-          hi = *(const v4i *)(p + 2);
-        }
-        rc[4 * g] = (v2i){lo.x, lo.y};
-        rc[4 * g + 1] = (v2i){lo.z, lo.w};
-        rc[4 * g + 2] = (v2i){hi.x, hi.y};
-        rc[4 * g + 3] = (v2i){hi.z, hi.w};
+    for (int g = 0; g < 4; g++) {
+      v4i lo = {0, 0, 0, 0}, hi = {0, 0, 0, 0};
+      if (first) {
+        const v2i *p = rowc + by * 32 + 8 * g + 4 * (lane >> 5);
+        lo = *(const v4i *)p;
+        hi = *(const v4i *)(p + 2);
       }
+      rc[4 * g] = (v2i){lo.x, lo.y};
+      rc[4 * g + 1] = (v2i){lo.z, lo.w};
+      rc[4 * g + 2] = (v2i){hi.x, hi.y};
+      rc[4 * g + 3] = (v2i){hi.z, hi.w};
+    }
 #pragma unroll
-      for (int j = 0; j < 2; j++) {
-        const int bx = bx0 + j;
-        if (bx >= by && bx < nb) {
-          const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
-          flush_block_f32_u4(vis + ((int64_t)f * NB + b) * 2048, re[i][j], im[i][j], U, mode,
-                             lane, rc, cc[j]);
-        }
+    for (int j = 0; j < 2; j++) {
+      const int bx = bx0 + j;
+      if (bx >= by && bx < nb) {
+        const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
+        flush_block_f32(vis + ((int64_t)f * NB + b) * 2048, re[i][j], im[i][j], U, mode, lane,
+                        rc, cc[j]);
       }
     }
-  } else {
-#pragma unroll
-    for (int i = 0; i < kG; i++)
-#pragma unroll
-      for (int j = 0; j < 2; j++) {
-        const int by = by0 + i, bx = bx0 + j;
-        if (bx >= by && bx < nb) {
-          const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
-          flush_block_f32(vis + ((int64_t)f * NB + b) * 2048, re[i][j], im[i][j], U, mode,
-                          lane);
-        }
-      }
   }
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
@@ -818,9 +679,9 @@ __global__ __launch_bounds__(256, 1) void corr_f6_kernel(GemmJob J, int head, in
   int ti_a, ti_b, ct;
   tile_coords((int)(item % per_f), J.ntiles, ti_a, ti_b, ct);
   if (ti_b >= 0 && ti_b != ti_a)
-    gemm_tile_f6<true, false, true>(J, (int)(item / per_f), ti_a, ti_b, ct, s0, ns, lds);
+    gemm_tile_f6<true>(J, (int)(item / per_f), ti_a, ti_b, ct, s0, ns, lds);
   else
-    gemm_tile_f6<false, false, true>(J, (int)(item / per_f), ti_a, ti_a, ct, s0, ns, lds);
+    gemm_tile_f6<false>(J, (int)(item / per_f), ti_a, ti_a, ct, s0, ns, lds);
 }
 
 // Batch: persistent workgroups take work items from 8 per-XCD queues (the
@@ -875,7 +736,8 @@ int launch_unpack_batch_fp6(const UnpackJob *d_jobs, int njobs, int64_t units, h
   if (units <= 0) return CHX_SUCCESS;
   const int64_t gx = units < 65536 ? units : 65536;
   const int64_t gy = (units + gx - 1) / gx;
-  unpack_f6_batch_kernel<<<dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s>>>(d_jobs, njobs);
+  unpack_f6_u4_batch_kernel<<<dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s>>>(d_jobs,
+                                                                                     njobs);
   CHX_HIP(hipGetLastError());
   return CHX_SUCCESS;
 }
@@ -900,7 +762,7 @@ static int64_t f6_chunk_samples(const chx_plan *p) {
   if (tc > desc) tc = desc;
   if (tc > (int64_t)kK * 32767) tc = (int64_t)kK * 32767;  // unpack runs <= 2 KB <= 65535
   // An array with at least a round of items (one per CU) takes chunks of at
-  // most kMaxStagesU4 stages, so that whole items cover a chunk (plain
+  // most kMaxStages stages, so that whole items cover a chunk (plain
   // stores); a longer chunk would cut every item into atomic pieces.
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->cfg.device) !=
@@ -908,8 +770,8 @@ static int64_t f6_chunk_samples(const chx_plan *p) {
       cus < 1)
     cus = 1;
   if ((int64_t)p->cfg.num_freq * paired_tiles(p->ntiles) >= cus &&
-      tc > (int64_t)kK * kMaxStagesU4)
-    tc = (int64_t)kK * kMaxStagesU4;
+      tc > (int64_t)kK * kMaxStages)
+    tc = (int64_t)kK * kMaxStages;
   tc = tc / unit * unit;
   const int64_t T = (p->cfg.time_steps + unit - 1) / unit * unit;
   if (tc > T) tc = T;
@@ -963,11 +825,11 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
     // take whole items (the chunk in one piece, plain stores); the items of
     // a last partial round are cut in time into the fewest pieces that
     // minimise its length and add atomically into zeroed output.  A chunk
-    // longer than kMaxStagesU4 cuts every item into pieces of at most
-    // kMaxStagesU4 stages (atomic).
+    // longer than kMaxStages cuts every item into pieces of at most
+    // kMaxStages stages (atomic).
     int64_t head = 0;  // items of the whole rounds
     int split = 1;
-    if (nst <= kMaxStagesU4) head = items / cus * cus;
+    if (nst <= kMaxStages) head = items / cus * cus;
     const int64_t tail = items - head;
     if (tail > 0) {
       double best = (double)((tail + cus - 1) / cus);
@@ -980,7 +842,7 @@ int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode
       }
     }
     int sps = ((nst + split - 1) / split + kRing - 1) / kRing * kRing;
-    if (sps > kMaxStagesU4) sps = kMaxStagesU4;
+    if (sps > kMaxStages) sps = kMaxStages;
     split = (nst + sps - 1) / sps;
     const int kmode = (t0 == 0) ? mode : 1;
     const int tmode = split > 1 ? 2 : kmode;  // the tail's pieces add atomically

@@ -78,11 +78,10 @@ int launch_u4_corrections(const uint32_t *partials, int32_t *corr, int F, int Np
 constexpr int kFp6Ring = 4;  // 64-sample stages in the LDS ring
 // Both matrix-core GEMMs use output tiles of 128 rows x 256 columns: row
 // tile ti holds ceil((ntiles - ti) / 2) tiles, the columns starting at row
-// tile ti.  A work item covers at most kMaxItemStages 64-sample stages, so
-// that every partial sum stays exact: FP6's FP32 sums (at most 128 * 64 per
-// stage) stay <= 2^24, int8's int32 sums (signed 64x products, at most
-// 8192 * 64 per stage) <= 2^30, or (offset-binary code products, at most
-// 450 * 64 per stage) < 2^26.
+// tile ti.  An int8 work item covers at most kMaxItemStages 64-sample
+// stages, so that its int32 sums stay exact: signed 64x products (at most
+// 8192 * 64 per stage) <= 2^30, offset-binary code products (at most 450 *
+// 64 per stage) < 2^26.  FP6 items: kMaxItemStagesFp6 (below).
 constexpr int kMaxItemStages = 2048;
 __host__ __device__ inline int64_t wide_tiles(int64_t ntiles) {
   const int64_t m = ntiles / 2;
@@ -111,8 +110,13 @@ inline int format_ring(int fmt) { return fmt == kPlanesFp6 ? kFp6Ring : kMfmaRin
 inline int64_t format_tiles(int, int64_t ntiles) { return paired_tiles(ntiles); }
 // work items of one channel on 128 x 128 tiles: the upper triangle of tiles
 __host__ __device__ inline int64_t small_tiles(int64_t ntiles) { return ntiles * (ntiles + 1) / 2; }
-// the most stages one work item may cover
-inline int format_max_stages(int) { return kMaxItemStages; }
+// the most stages one work item may cover: FP6 work items hold offset-binary
+// code products, at most 450 per sample, from -2^24: 1024 stages keep every
+// partial sum within 2^24 (int8: 2048)
+constexpr int kMaxItemStagesFp6 = 1024;
+inline int format_max_stages(int fmt) {
+  return fmt == kPlanesFp6 ? kMaxItemStagesFp6 : kMaxItemStages;
+}
 // stages per work item of an nst-stage chunk cut at max_st (0 = whole)
 inline int format_item_stages(int fmt, int nst, int max_st) {
   int per = (max_st > 0 && max_st < nst) ? max_st : nst;
@@ -149,10 +153,14 @@ struct GemmItem {
   int job, f, tile, stage0, nstages;
 };
 // One job's chunk for the batched unpack, starting at global workgroup
-// unit0 (increasing over jobs): units = ceil(Np/1024) * KB * F (int8) or
-// ceil(Np/1024) * ceil(2 KB / fp6_halves_per_unit(Np)) * F (fp6: one
-// workgroup per 32-sample half, or per several for arrays under 1024).  Its
-// KB blocks land at blocks [kb_dst0, kb_dst0 + KB) of planes laid out with
+// unit0 (increasing over jobs).  int8: ceil(Np/1024) * KB * F workgroups,
+// one per 16-sample block of 1024 elements.  FP6 (offset-binary codes):
+// ceil(Np/1024) * runs * F workgroups; the job's units (1024 elements x one
+// 32-sample half, or for arrays under 1024 elements several halves of all
+// elements) go round-robin over its runs, and run r writes its elements'
+// code sums into row prow0 + r of `partials` (rows [F][Np] u32, SR | SI <<
+// 16), the partial rows of the GEMM job the chunk belongs to.  Its KB
+// blocks land at blocks [kb_dst0, kb_dst0 + KB) of planes laid out with
 // KB_dst blocks (format layouts above).  t_valid = 0 writes zeros and reads
 // nothing.
 struct UnpackJob {
@@ -160,16 +168,40 @@ struct UnpackJob {
   uint8_t *planes, *planes1;
   int64_t row_bytes, t0, t_valid, unit0;
   int N, F, Np, KB, KB_dst, kb_dst0;
+  uint32_t *partials;  // FP6 only
+  int prow0, runs;     // FP6 only
 };
 // 32-sample halves per fp6 unpack workgroup: arrays under 1024 elements
 // take several, so that all 1024 element slots work
 __host__ __device__ inline int fp6_halves_per_unit(int Np) { return Np >= 1024 ? 1 : 1024 / Np; }
+// FP6 unpack units of a chunk of KB 64-sample blocks
+__host__ __device__ inline int64_t f6_u4_units(int64_t KB, int Np) {
+  const int hpu = fp6_halves_per_unit(Np);
+  return (2 * KB + hpu - 1) / hpu;
+}
+// Runs (workgroups per channel and element group) of an FP6 batch unpack
+// job: 16 units each, so that a run's code sums (at most 15 * 32 * 8 * 16
+// per element) fit 16 bits.
+__host__ __device__ inline int f6_u4_job_runs(int64_t KB, int Np) {
+  return (int)((f6_u4_units(KB, Np) + 15) / 16);
+}
 inline int64_t unpack_units(int fmt, const UnpackJob &u) {
   const int64_t groups = (u.Np + 1023) / 1024;
   if (fmt != kPlanesFp6) return groups * u.KB * u.F;
-  const int hpu = fp6_halves_per_unit(u.Np);
-  return groups * ((2 * (int64_t)u.KB + hpu - 1) / hpu) * u.F;
+  return groups * u.runs * u.F;
 }
+// The offset corrections of one GEMM job (FP6 batch and serve): partials
+// [nrun][F][Np] of code sums over a chunk of n valid samples -> corr [2][F]
+// [Np] int32 pairs (rows, then columns), as launch_u4_corrections; the
+// job's workgroups start at blk0 (64 elements each, increasing over jobs).
+struct CorrJob {
+  const uint32_t *partials;
+  int32_t *corr;
+  int64_t blk0;
+  int F, Np, nrun, n;
+};
+int launch_u4_corrections_batch(const CorrJob *d_jobs, int njobs, int64_t blocks,
+                                hipStream_t s);
 int launch_unpack_batch(int fmt, const UnpackJob *d_jobs, int njobs, int64_t units,
                         hipStream_t s);
 // Work items of one (job, channel) group; groups are spread over the 8

@@ -320,13 +320,13 @@ __global__ __launch_bounds__(256) void unpack_u4_kernel(
 // corr[1][f][e] = (-8 (SR + SI), 8 (SR - SI)) for its columns.  Workgroup =
 // 64 elements x 4 run quarters (lanes of a wave read 64 consecutive
 // partials), added up through LDS.
-__global__ __launch_bounds__(256) void u4_corrections_kernel(const uint32_t *__restrict__ partials,
-                                                             int32_t *__restrict__ corr, int F,
-                                                             int Np, int nrun, int32_t n) {
+__device__ __forceinline__ void u4_corrections_block(const uint32_t *__restrict__ partials,
+                                                     int32_t *__restrict__ corr, int F, int Np,
+                                                     int nrun, int32_t n, int64_t blk) {
   __shared__ uint32_t part[2][4][64];
   const int64_t FN = (int64_t)F * Np;
   const int lane = threadIdx.x & 63, quarter = threadIdx.x >> 6;
-  const int64_t i = (int64_t)blockIdx.x * 64 + lane;  // (f, e) flattened
+  const int64_t i = blk * 64 + lane;  // (f, e) flattened
   uint32_t sr = 0, si = 0;
   if (i < FN) {
     for (int r = quarter; r < nrun; r += 4) {
@@ -346,6 +346,28 @@ __global__ __launch_bounds__(256) void u4_corrections_kernel(const uint32_t *__r
     ((v2i *)corr)[i] = (v2i){-8 * (SR + SI) + 128 * n, -8 * SR - 7 * SI};
     ((v2i *)corr)[FN + i] = (v2i){-8 * (SR + SI), 8 * (SR - SI)};
   }
+}
+
+__global__ __launch_bounds__(256) void u4_corrections_kernel(const uint32_t *__restrict__ partials,
+                                                             int32_t *__restrict__ corr, int F,
+                                                             int Np, int nrun, int32_t n) {
+  u4_corrections_block(partials, corr, F, Np, nrun, n, blockIdx.x);
+}
+
+// Many jobs in one launch: workgroup g belongs to the job j with
+// jobs[j].blk0 <= g < jobs[j+1].blk0 (binary search).
+__global__ __launch_bounds__(256) void u4_corrections_batch_kernel(
+    const CorrJob *__restrict__ jobs, int njobs) {
+  const int64_t g = blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].blk0 <= g) lo = mid; else hi = mid - 1;
+  }
+  const CorrJob &J = jobs[lo];
+  const int64_t blk = g - J.blk0;
+  if (blk * 64 >= (int64_t)J.F * J.Np) return;  // grid padding (uniform per workgroup)
+  u4_corrections_block(J.partials, J.corr, J.F, J.Np, J.nrun, J.n, blk);
 }
 
 // Batch: one launch unpacks the chunks of many jobs.  Workgroup g belongs
@@ -875,6 +897,17 @@ int launch_u4_corrections(const uint32_t *partials, int32_t *corr, int F, int Np
   const int64_t FN = (int64_t)F * Np;
   u4_corrections_kernel<<<dim3((unsigned)((FN + 63) / 64)), dim3(256), 0, s>>>(partials, corr, F,
                                                                              Np, nrun, n);
+  CHX_HIP(hipGetLastError());
+  return CHX_SUCCESS;
+}
+
+int launch_u4_corrections_batch(const CorrJob *d_jobs, int njobs, int64_t blocks,
+                                hipStream_t s) {
+  if (blocks <= 0) return CHX_SUCCESS;
+  const int64_t gx = blocks < 65536 ? blocks : 65536;
+  const int64_t gy = (blocks + gx - 1) / gx;
+  u4_corrections_batch_kernel<<<dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s>>>(d_jobs,
+                                                                                       njobs);
   CHX_HIP(hipGetLastError());
   return CHX_SUCCESS;
 }

@@ -74,8 +74,25 @@ int64_t group_stages(int fmt, const Stream &s, int64_t k) {
   const int64_t kb = group_blocks(fmt, s, k);
   return fmt == chx::kPlanesFp6 ? kb : kb / 4;
 }
-int64_t group_bytes(int fmt, const Stream &s, int64_t k) {
+int64_t group_plane_bytes(int fmt, const Stream &s, int64_t k) {
   return chx::format_plane_bytes(fmt, s.F, s.Np, group_stages(fmt, s, k));
+}
+// FP6 code-sum rows of k frames side by side: each frame's unpack runs,
+// then the ring padding's.
+int64_t group_rows(const Stream &s, int64_t k) {
+  const int64_t pad = group_blocks(chx::kPlanesFp6, s, k) - k * s.KBf;
+  return k * chx::f6_u4_job_runs(s.KBf, (int)s.Np) +
+         (pad > 0 ? chx::f6_u4_job_runs(pad, (int)s.Np) : 0);
+}
+int64_t group_partial_bytes(const Stream &s, int64_t k) {
+  return round_up(group_rows(s, k) * s.F * s.Np * 4, 256);
+}
+// Workspace of k frames of a stream: planes, and (FP6) code-sum rows and
+// offset corrections.
+int64_t group_bytes(int fmt, const Stream &s, int64_t k) {
+  int64_t b = group_plane_bytes(fmt, s, k);
+  if (fmt == chx::kPlanesFp6) b += group_partial_bytes(s, k) + round_up(2 * s.F * s.Np * 8, 256);
+  return b;
 }
 // One (array, plane, channel) region of a group must stay addressable by a
 // 32-bit buffer descriptor.
@@ -137,9 +154,10 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
     m.push_back(i);
   }
   std::vector<UnpackJob> ujobs;
+  std::vector<CorrJob> cjobs;
   std::vector<GemmJob> gjobs;
   std::vector<ItemGroup> groups;
-  int64_t units = 0, off = 0;
+  int64_t units = 0, off = 0, cblocks = 0;
   for (int64_t sid : order) {
     const std::vector<int> &m = members[sid];
     const Stream &s = *st[m[0]];
@@ -147,7 +165,13 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
     const int KBt = (int)group_blocks(fmt, s, (int64_t)m.size());
     uint8_t *planes = S->d_work + off;
     uint8_t *planes1 = fmt == kPlanesFp6 ? planes + 64LL * s.F * KBt * s.Np : nullptr;
+    const bool codes = fmt == kPlanesFp6;  // FP6: offset-binary codes and corrections
+    const int64_t pbytes = group_plane_bytes(fmt, s, (int64_t)m.size());
+    uint32_t *partials = codes ? (uint32_t *)(planes + pbytes) : nullptr;
+    int32_t *corr =
+        codes ? (int32_t *)(planes + pbytes + group_partial_bytes(s, (int64_t)m.size())) : nullptr;
     off += group_bytes(fmt, s, (int64_t)m.size());
+    int prow = 0;
     for (int k = 0; k < (int)m.size(); k++) {
       UnpackJob u;
       u.in = batch[m[k]].src;
@@ -163,6 +187,10 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
       u.KB = s.KBf;
       u.KB_dst = KBt;
       u.kb_dst0 = k * s.KBf;
+      u.partials = partials;
+      u.prow0 = prow;
+      u.runs = codes ? f6_u4_job_runs(u.KB, (int)s.Np) : 0;
+      prow += u.runs;
       ujobs.push_back(u);
       units += unpack_units(fmt, u);
     }
@@ -173,8 +201,24 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
       z.unit0 = units;
       z.KB = KBt - used;
       z.kb_dst0 = used;
+      z.prow0 = prow;
+      z.runs = codes ? f6_u4_job_runs(z.KB, (int)s.Np) : 0;
+      prow += z.runs;
       ujobs.push_back(z);
       units += unpack_units(fmt, z);
+    }
+    if (codes) {
+      if (prow != group_rows(s, (int64_t)m.size())) return CHX_ERR_HIP;
+      CorrJob c;
+      c.partials = partials;
+      c.corr = corr;
+      c.blk0 = cblocks;
+      c.F = s.F;
+      c.Np = (int)s.Np;
+      c.nrun = prow;
+      c.n = (int)((int64_t)m.size() * s.T);  // valid samples (padding adds nothing)
+      cjobs.push_back(c);
+      cblocks += ((int64_t)s.F * s.Np + 63) / 64;
     }
     const int nst = (int)group_stages(fmt, s, (int64_t)m.size());
     const int pieces = add_job_items(fmt, (int)gjobs.size(), s.F, s.ntiles, nst,
@@ -193,7 +237,7 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
     g.mode = pieces > 1 ? 2 : 1;  // accumulate (atomically when a tile is cut in time)
     g.stages_per_split = nst;
     g.small = 0;
-    g.corr = nullptr;
+    g.corr = corr;
     gjobs.push_back(g);
   }
   std::vector<GemmItem> items;
@@ -206,8 +250,10 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
                                (size_t)S->cfg.workspace_bytes);
   if (rc) return rc;
 
-  // One table upload: [ujobs][gjobs][items][qbound], 256-byte aligned parts.
-  const size_t a0 = 0, a1 = round_up(a0 + ujobs.size() * sizeof(UnpackJob), 256),
+  // One table upload: [ujobs][cjobs][gjobs][items][qbound], 256-byte aligned
+  // parts.
+  const size_t a0 = 0, ac = round_up(a0 + ujobs.size() * sizeof(UnpackJob), 256),
+               a1 = round_up(ac + cjobs.size() * sizeof(CorrJob), 256),
                a2 = round_up(a1 + gjobs.size() * sizeof(GemmJob), 256),
                a3 = round_up(a2 + items.size() * sizeof(GemmItem), 256),
                end = a3 + qbound.size() * sizeof(int);
@@ -223,6 +269,7 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
     S->tab_cap = cap;
   }
   memcpy(S->h_tab + a0, ujobs.data(), ujobs.size() * sizeof(UnpackJob));
+  if (!cjobs.empty()) memcpy(S->h_tab + ac, cjobs.data(), cjobs.size() * sizeof(CorrJob));
   memcpy(S->h_tab + a1, gjobs.data(), gjobs.size() * sizeof(GemmJob));
   memcpy(S->h_tab + a2, items.data(), items.size() * sizeof(GemmItem));
   memcpy(S->h_tab + a3, qbound.data(), qbound.size() * sizeof(int));
@@ -232,6 +279,9 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
   if (!rc)
     rc = launch_unpack_batch(S->fmt, (const UnpackJob *)(S->d_tab + a0), (int)ujobs.size(),
                              units, cs);
+  if (!rc && !cjobs.empty())
+    rc = launch_u4_corrections_batch((const CorrJob *)(S->d_tab + ac), (int)cjobs.size(), cblocks,
+                                     cs);
   if (!rc)
     rc = launch_gemm_batch(S->fmt, (const GemmJob *)(S->d_tab + a1),
                            (const GemmItem *)(S->d_tab + a2),

@@ -249,8 +249,8 @@ def _small_item_blocks(nt):
 
 def _expected_items(jobs, chunk, max_st, unit, method, small=0):
     """Work items of the plan: per round, per active job, F x tiles x pieces
-    (pieces hold at most 2048 stages; FP6 jobs of at most `small` elements
-    on 128 x 128 tiles)."""
+    (pieces hold at most 2048 stages, FP6 1024; FP6 jobs of at most `small`
+    elements on 128 x 128 tiles)."""
     n, rounds = 0, -(-max(j["time_steps"] for j in jobs) // chunk)
     for r in range(rounds):
         for j in jobs:
@@ -259,7 +259,7 @@ def _expected_items(jobs, chunk, max_st, unit, method, small=0):
                 continue
             nst = -(-min(chunk, j["time_steps"] - t0) // unit) * (unit // 64)
             per = max_st if 0 < max_st < nst else nst
-            per = min(per, 2048)
+            per = min(per, 1024 if method == 3 else 2048)
             nt = -(-j["num_elements"] // 128)
             tiles = (nt * (nt + 1) // 2 if method == 3 and j["num_elements"] <= small
                      else _tiles(nt, method))
@@ -343,15 +343,16 @@ def test_batch_plan_host_only(tuning, method):
                                        if method == 3 else 0)
 
 
-@pytest.mark.parametrize("method", [2, 3])
-def test_items_hold_at_most_2048_stages(method):
-    """A long job is cut into work items of <= 2048 stages (FP6's FP32
-    partial sums stay <= 2^24, where they are exact; int8's int32 sums of 64x
-    products <= 2^30)."""
+@pytest.mark.parametrize("method, cap", [(2, 2048), (3, 1024)])
+def test_items_hold_at_most_their_cap(method, cap):
+    """A long job is cut into work items of <= 2048 stages (int8: its int32
+    sums of 64x products stay <= 2^30) or <= 1024 (FP6: its FP32 sums of
+    code products, from -2^24, stay within 2^24, where they are exact)."""
     job = [dict(num_elements=256, num_freq=3, time_steps=5000 * 64)]  # 5000 stages
     plan = chx.batch_plan(_fake(job), dict(workspace_bytes=64 << 30, method=method))
     assert plan["rounds"] == 1 and plan["chunk_samples"] == 5000 * 64
-    assert plan["items"] == 3 * 2 * 3  # channels x tiles (nt = 2) x ceil(5000 / 2048) pieces
+    # channels x tiles (nt = 2) x ceil(5000 / cap) pieces
+    assert plan["items"] == 3 * 2 * -(-5000 // cap)
 
 
 def test_batch_plan_validates_like_create():

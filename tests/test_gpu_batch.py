@@ -198,33 +198,38 @@ def test_uneven_batch_64_jobs_matches_alone_and_oracle(method, small):
         _pairs_check(torch, ins[k], outs[k], N, U, rng)
 
 
-def test_fp6_item_at_the_exact_bound():
-    """Whole-chunk FP6 items of 2048 stages with every sample (-8, -8): each
-    FP32 accumulator reaches 128 * 2048 * 64 = 2^24 (the largest value it
-    holds exactly), and the one-ring rest adds atomically."""
+@pytest.mark.parametrize("byte, re_per_sample", [(0xFF, 98), (0x00, 128)])
+def test_fp6_items_at_the_code_bound(byte, re_per_sample):
+    """FP6 items of 1024 stages (the most one holds) with every sample (7,7)
+    (codes 15: every accumulator climbs 450 per sample from -2^24 to
+    -2^24 + 450 * 65536, the largest value it holds) or (-8,-8) (codes 0:
+    the accumulators stay at -2^24 and the offset corrections carry the
+    whole result), 2052 stages in pieces of 1024 + 1024 + 4 that add
+    atomically: re = 98 T or 128 T, im = 0."""
     torch = _torch()
     T, F, N = 2048 * 64 + 256, 2, 128
-    x = torch.zeros((T, F, N), dtype=torch.uint8, device="cuda")
+    x = torch.full((T, F, N), byte, dtype=torch.uint8, device="cuda")
     vis = torch.full((F, chx.num_blocks(N), 32, 32, 2), 3, dtype=torch.int32, device="cuda")
     job = dict(num_elements=N, num_freq=F, time_steps=T, upper_triangle_convention=1,
                d_input=x.data_ptr(), d_vis=vis.data_ptr())
     with chx.Batch([job], dict(method=chx.FP6_MFMA, workspace_bytes=1 << 30)) as b:
         inf = b.info()
         say(f"  bound batch: {inf}")
-        assert inf["rounds"] == 1 and inf["items"] == F * 1 * 2
+        assert inf["rounds"] == 1 and inf["items"] == F * 1 * 3
         b.run()
         torch.cuda.synchronize()
     v = vis.cpu().numpy()
-    assert (v[..., 0] == 128 * T).all() and (v[..., 1] == 0).all()
+    assert (v[..., 0] == re_per_sample * T).all() and (v[..., 1] == 0).all()
 
 
 def test_fp6_item_near_the_bound_with_mixed_signs():
-    """A whole-chunk FP6 item of 2048 stages whose accumulators run near
-    2^24 with products of both signs: for the first 2047 stages, elements of
-    a flip set alternate per sample between (-8,-8) and (7,7) (products
-    +128 / -112 against the others, +128 / +98 among themselves) and the
-    rest hold (-8,-8) (+128 per sample); the last stage and the one-ring
-    rest are uniform random.  FP6 equals the int8 method and the oracle."""
+    """Visibilities near 2^24 with products of both signs: for the first
+    2047 stages, elements of a flip set alternate per sample between
+    (-8,-8) and (7,7) (products +128 / -112 against the others, +128 / +98
+    among themselves) and the rest hold (-8,-8) (+128 per sample); the last
+    stage and the one-ring rest are uniform random.  FP6 (items of 1024
+    stages on offset-binary codes, pieces added atomically) equals the int8
+    method (one item of 2048 stages) and the oracle."""
     torch = _torch()
     T, F, N = 2048 * 64 + 256, 1, 128
     t_const = 2047 * 64
@@ -245,7 +250,8 @@ def test_fp6_item_near_the_bound_with_mixed_signs():
         with chx.Batch([job], dict(method=method, workspace_bytes=1 << 30)) as b:
             inf = b.info()
             say(f"  near-bound batch {chx.METHOD_NAMES[method]}: {inf}")
-            assert inf["rounds"] == 1 and inf["items"] == F * 1 * 2
+            assert inf["rounds"] == 1 and inf["items"] == F * 1 * (3 if method == chx.FP6_MFMA
+                                                                 else 2)
             b.run()
             torch.cuda.synchronize()
         got[method] = vis.cpu().numpy()
