@@ -325,12 +325,15 @@ int plan_sizes(chx_batch *b, std::vector<JobOffsets> &off) {
   }
   if (b->fmt == kPlanesFp6) {
     for (int j = 0; j < nj; j++) {
-      const int64_t F = b->jobs[j].num_freq;
-      const int64_t stages = std::min(chunk, round_up(b->jobs[j].time_steps, unit)) / kMfmaStage;
+      const int64_t F = b->jobs[j].num_freq, T = b->jobs[j].time_steps;
+      // the most partial rows any round of the job writes
+      int64_t rows = 0;
+      for (int64_t t0 = 0; t0 < T; t0 += chunk) {
+        const int64_t stages = round_up(std::min(chunk, T - t0), unit) / kMfmaStage;
+        rows = std::max<int64_t>(rows, f6_u4_job_runs(format_kb(b->fmt, (int)stages), (int)Np[j]));
+      }
       off[j].partials = ws;
-      ws += round_up((int64_t)f6_u4_job_runs(format_kb(b->fmt, (int)stages), (int)Np[j]) * F *
-                         Np[j] * 4,
-                     256);
+      ws += round_up(rows * F * Np[j] * 4, 256);
       off[j].corr = ws;
       ws += round_up(2 * F * Np[j] * 8, 256);
     }

@@ -180,10 +180,14 @@ __host__ __device__ inline int64_t f6_u4_units(int64_t KB, int Np) {
   return (2 * KB + hpu - 1) / hpu;
 }
 // Runs (workgroups per channel and element group) of an FP6 batch unpack
-// job: 16 units each, so that a run's code sums (at most 15 * 32 * 8 * 16
-// per element) fit 16 bits.
+// job: a run takes units/8 units, at least 1 and at most 16 (so that its
+// code sums, at most 15 * 32 * 8 * 16 per element, fit 16 bits); a small
+// job (a streamed frame: 32 units) still spreads over many workgroups.
 __host__ __device__ inline int f6_u4_job_runs(int64_t KB, int Np) {
-  return (int)((f6_u4_units(KB, Np) + 15) / 16);
+  const int64_t units = f6_u4_units(KB, Np);
+  int64_t per = units / 8;
+  per = per < 1 ? 1 : (per > 16 ? 16 : per);
+  return (int)((units + per - 1) / per);
 }
 inline int64_t unpack_units(int fmt, const UnpackJob &u) {
   const int64_t groups = (u.Np + 1023) / 1024;
