@@ -37,8 +37,9 @@ def signature(jobs) -> str:
     return chx.job_signature(jobs, chx.device_arch(0))
 
 
-def candidates(cus: int):
-    methods = (3, 2)  # FP6, int8 matrix cores (bit-identical results)
+def candidates(cus: int, methods=(3, 2)):
+    """Launch shapes to time; methods: 3 = FP6, 2 = int8 matrix cores
+    (bit-identical results)."""
     stages = (0, 1024, 512, 256, 128)  # whole chunk (FP6: <= 2048 stages), or pieces
     wgs = (cus, cus * 7 // 8 // 8 * 8, cus // 2 // 8 * 8)
     budgets = (16 << 30, 64 << 30)
@@ -69,11 +70,11 @@ def time_batch(descs, tuning, reps: int = 3) -> float:
     return ts[len(ts) // 2]
 
 
-def tune(jobs, descs, reps: int = 3, log=print) -> dict:
+def tune(jobs, descs, reps: int = 3, log=print, methods=(3, 2)) -> dict:
     cus = torch.cuda.get_device_properties(0).multi_processor_count // 8 * 8
     total = sum(W.cmacs(j["num_elements"], j["num_freq"], j["time_steps"]) for j in jobs)
     table = []
-    for c in candidates(cus):
+    for c in candidates(cus, methods):
         try:
             ms = time_batch(descs, c, reps)
         except chx.ChxError as e:  # e.g. a budget the device cannot hold
@@ -108,6 +109,8 @@ def main() -> int:
     ap.add_argument("--out", default=DEFAULT_CACHE)
     ap.add_argument("--sizes", default="256,512,1024,2048",
                     help="array sizes the job list draws from")
+    ap.add_argument("--methods", default="3,2",
+                    help="matrix-core methods to time (3 = FP6, 2 = int8)")
     ap.add_argument("--replay", type=int, default=0,
                     help="no tuning: replay the cached shape this many times and report it")
     a = ap.parse_args()
@@ -130,7 +133,8 @@ def main() -> int:
         total = sum(W.cmacs(j["num_elements"], j["num_freq"], j["time_steps"]) for j in jobs)
         print(json.dumps(dict(tuning=tuning, ms=ms, tcmacs=total / ms / 1e9)), flush=True)
         return 0
-    res = tune(jobs, descs, a.reps, log=lambda m: print(m, flush=True))
+    res = tune(jobs, descs, a.reps, log=lambda m: print(m, flush=True),
+               methods=tuple(int(x) for x in a.methods.split(",")))
     save(res, a.out)
     print(json.dumps(res["best"]), flush=True)
     return 0

@@ -174,17 +174,18 @@ int check_launch_tables(int fmt, const std::vector<UnpackJob> &ujobs,
         u.t_valid > (int64_t)spb * u.KB || u.Np % kMfmaTile || u.N > u.Np || u.kb_dst0 < 0 ||
         u.kb_dst0 + u.KB > u.KB_dst)
       return CHX_ERR_HIP;
-    if (fmt == kPlanesFp6) {
-      // code sums: runs of at most 16 units (16-bit sums), inside their
-      // region; the unit rows of one input unit fit 32-bit offsets
-      if (u.planes1 != u.planes + 64LL * u.F * u.KB_dst * u.Np || !u.partials || u.prow0 < 0 ||
-          u.runs < 1 || (int64_t)u.runs * 16 < f6_u4_units(u.KB, u.Np) ||
-          u.row_bytes * (32LL * fp6_halves_per_unit(u.Np) + 1) > 0x7FFFFFFFLL)
-        return CHX_ERR_HIP;
-      const int64_t rows = (int64_t)(u.prow0 + u.runs) * u.F * u.Np * 4;
-      if ((const uint8_t *)u.partials < base || (const uint8_t *)u.partials + rows > end)
-        return CHX_ERR_HIP;
-    }
+    if (fmt == kPlanesFp6 && u.planes1 != u.planes + 64LL * u.F * u.KB_dst * u.Np)
+      return CHX_ERR_HIP;
+    // code sums: runs of at most 16 units (16-bit sums), inside their
+    // region; the sample rows of one unit fit 32-bit offsets
+    const int64_t unit_rows = fmt == kPlanesFp6 ? 32LL * fp6_halves_per_unit(u.Np) : 16;
+    if (!u.partials || u.prow0 < 0 || u.runs < 1 ||
+        (int64_t)u.runs * 16 < u4_units(fmt, u.KB, u.Np) ||
+        u.row_bytes * (unit_rows + 1) > 0x7FFFFFFFLL)
+      return CHX_ERR_HIP;
+    const int64_t rows = (int64_t)(u.prow0 + u.runs) * u.F * u.Np * 4;
+    if ((const uint8_t *)u.partials < base || (const uint8_t *)u.partials + rows > end)
+      return CHX_ERR_HIP;
   }
   for (const GemmJob &g : gjobs) {
     const int stages = fmt == kPlanesFp6 ? g.KB : g.KB / 4;
@@ -194,9 +195,8 @@ int check_launch_tables(int fmt, const std::vector<UnpackJob> &ujobs,
         format_kb(fmt, stages) != g.KB || g.ntiles * kMfmaTile != g.Np || region > 0x7FFFFFFFLL)
       return CHX_ERR_HIP;
     if (fmt == kPlanesFp6 && g.planes1 != g.planes + 64LL * g.F * g.KB * g.Np) return CHX_ERR_HIP;
-    if (fmt == kPlanesFp6 &&
-        (!g.corr || (const uint8_t *)g.corr < base ||
-         (const uint8_t *)g.corr + 16LL * g.F * g.Np > end))
+    if (!g.corr || (const uint8_t *)g.corr < base ||
+        (const uint8_t *)g.corr + 16LL * g.F * g.Np > end)
       return CHX_ERR_HIP;
     if (g.small && fmt != kPlanesFp6) return CHX_ERR_HIP;
   }
@@ -315,29 +315,42 @@ int plan_sizes(chx_batch *b, std::vector<JobOffsets> &off) {
     const int64_t lim = b->fmt == kPlanesFp6 ? 0x7FFFFFFFLL / (Np[j] / 2) : 0x7FFFFFFFLL / Np[j];
     chunk = std::min<int64_t>(chunk, lim / unit * unit);
   }
-  b->chunk = chunk;
-  b->rounds = (int)((maxT + chunk - 1) / chunk);
-  int64_t ws = 0;
-  for (int j = 0; j < nj; j++) {
-    off[j].planes = ws;
-    const int64_t stages = std::min(chunk, round_up(b->jobs[j].time_steps, unit)) / kMfmaStage;
-    ws += format_plane_bytes(b->fmt, b->jobs[j].num_freq, Np[j], stages);
-  }
-  if (b->fmt == kPlanesFp6) {
+  // Layout of a chunk: every job's planes, then its code-sum partial rows
+  // (the most any of its rounds writes) and offset corrections.  The chunk
+  // shrinks until the whole layout fits the budget.
+  auto layout = [&](int64_t c) {
+    int64_t ws = 0;
+    for (int j = 0; j < nj; j++) {
+      off[j].planes = ws;
+      const int64_t stages = std::min(c, round_up(b->jobs[j].time_steps, unit)) / kMfmaStage;
+      ws += format_plane_bytes(b->fmt, b->jobs[j].num_freq, Np[j], stages);
+    }
     for (int j = 0; j < nj; j++) {
       const int64_t F = b->jobs[j].num_freq, T = b->jobs[j].time_steps;
-      // the most partial rows any round of the job writes
       int64_t rows = 0;
-      for (int64_t t0 = 0; t0 < T; t0 += chunk) {
-        const int64_t stages = round_up(std::min(chunk, T - t0), unit) / kMfmaStage;
-        rows = std::max<int64_t>(rows, f6_u4_job_runs(format_kb(b->fmt, (int)stages), (int)Np[j]));
+      for (int64_t t0 = 0; t0 < T; t0 += c) {
+        const int64_t stages = round_up(std::min(c, T - t0), unit) / kMfmaStage;
+        rows = std::max<int64_t>(rows,
+                                 u4_job_runs(b->fmt, format_kb(b->fmt, (int)stages), (int)Np[j]));
       }
       off[j].partials = ws;
       ws += round_up(rows * F * Np[j] * 4, 256);
       off[j].corr = ws;
       ws += round_up(2 * F * Np[j] * 8, 256);
     }
+    return ws;
+  };
+  const int64_t budget = b->tune.workspace_bytes;
+  int64_t ws = layout(chunk);
+  while (ws > budget && chunk > unit) {
+    int64_t c = (int64_t)((double)chunk * budget / ws) / unit * unit;
+    if (c >= chunk) c = chunk - unit;
+    chunk = std::max(c, unit);
+    ws = layout(chunk);
   }
+  if (ws > budget) return CHX_ERR_BUDGET;
+  b->chunk = chunk;
+  b->rounds = (int)((maxT + chunk - 1) / chunk);
   b->work_bytes = (size_t)ws;
   return CHX_SUCCESS;
 }
@@ -372,9 +385,8 @@ int plan_tables(chx_batch *b, uint8_t *base, const std::vector<JobOffsets> &off,
       const int nt = (N + kMfmaTile - 1) / kMfmaTile;
       const int64_t Npj = round_up(N, kMfmaTile);
       uint8_t *planes = base + off[j].planes;
-      const bool codes = b->fmt == kPlanesFp6;  // FP6: offset-binary codes and corrections
-      uint32_t *partials = codes ? (uint32_t *)(base + off[j].partials) : nullptr;
-      int32_t *corr = codes ? (int32_t *)(base + off[j].corr) : nullptr;
+      uint32_t *partials = (uint32_t *)(base + off[j].partials);
+      int32_t *corr = (int32_t *)(base + off[j].corr);
       uint8_t *planes1 = b->fmt == kPlanesFp6 ? planes + 64LL * F * KB * Npj : nullptr;
 
       UnpackJob u;
@@ -393,10 +405,10 @@ int plan_tables(chx_batch *b, uint8_t *base, const std::vector<JobOffsets> &off,
       u.kb_dst0 = 0;
       u.partials = partials;
       u.prow0 = 0;
-      u.runs = codes ? f6_u4_job_runs(KB, (int)Npj) : 0;
+      u.runs = u4_job_runs(b->fmt, KB, (int)Npj);
       ujobs.push_back(u);
       R.units += unpack_units(b->fmt, u);
-      if (codes) {
+      {
         CorrJob c;
         c.partials = partials;
         c.corr = corr;
@@ -476,7 +488,7 @@ int enqueue(const chx_batch *b, hipStream_t s) {
   for (int r = 0; r < b->rounds && !rc; r++) {
     const chx_batch::Round &R = b->round[r];
     rc = launch_unpack_batch(b->fmt, b->d_ujobs + R.ujob0, R.nujobs, R.units, s);
-    if (!rc && R.ncjobs > 0)
+    if (!rc)
       rc = launch_u4_corrections_batch(b->d_cjobs + R.cjob0, R.ncjobs, R.cblocks, s);
     if (!rc)
       rc = launch_gemm_batch(b->fmt, b->d_gjobs, b->d_items, b->d_qbound + 9 * r,

@@ -79,9 +79,8 @@ constexpr int kFp6Ring = 4;  // 64-sample stages in the LDS ring
 // Both matrix-core GEMMs use output tiles of 128 rows x 256 columns: row
 // tile ti holds ceil((ntiles - ti) / 2) tiles, the columns starting at row
 // tile ti.  An int8 work item covers at most kMaxItemStages 64-sample
-// stages, so that its int32 sums stay exact: signed 64x products (at most
-// 8192 * 64 per stage) <= 2^30, offset-binary code products (at most 450 *
-// 64 per stage) < 2^26.  FP6 items: kMaxItemStagesFp6 (below).
+// stages: its int32 sums of offset-binary code products (at most 450 * 64
+// per stage) stay < 2^26.  FP6 items: kMaxItemStagesFp6 (below).
 constexpr int kMaxItemStages = 2048;
 __host__ __device__ inline int64_t wide_tiles(int64_t ntiles) {
   const int64_t m = ntiles / 2;
@@ -152,14 +151,14 @@ inline int64_t job_tiles(int fmt, const GemmJob &g) {
 struct GemmItem {
   int job, f, tile, stage0, nstages;
 };
-// One job's chunk for the batched unpack, starting at global workgroup
-// unit0 (increasing over jobs).  int8: ceil(Np/1024) * KB * F workgroups,
-// one per 16-sample block of 1024 elements.  FP6 (offset-binary codes):
-// ceil(Np/1024) * runs * F workgroups; the job's units (1024 elements x one
-// 32-sample half, or for arrays under 1024 elements several halves of all
-// elements) go round-robin over its runs, and run r writes its elements'
-// code sums into row prow0 + r of `partials` (rows [F][Np] u32, SR | SI <<
-// 16), the partial rows of the GEMM job the chunk belongs to.  Its KB
+// One job's chunk for the batched unpack (offset-binary codes), starting at
+// global workgroup unit0 (increasing over jobs): ceil(Np/1024) * runs * F
+// workgroups.  The job's units (int8: 1024 elements x one 16-sample block;
+// FP6: 1024 elements x one 32-sample half, or for arrays under 1024
+// elements several halves of all elements) go round-robin over its runs,
+// and run r writes its elements' code sums into row prow0 + r of
+// `partials` (rows [F][Np] u32, SR | SI << 16), the partial rows of the
+// GEMM job the chunk belongs to.  Its KB
 // blocks land at blocks [kb_dst0, kb_dst0 + KB) of planes laid out with
 // KB_dst blocks (format layouts above).  t_valid = 0 writes zeros and reads
 // nothing.
@@ -168,8 +167,8 @@ struct UnpackJob {
   uint8_t *planes, *planes1;
   int64_t row_bytes, t0, t_valid, unit0;
   int N, F, Np, KB, KB_dst, kb_dst0;
-  uint32_t *partials;  // FP6 only
-  int prow0, runs;     // FP6 only
+  uint32_t *partials;
+  int prow0, runs;
 };
 // 32-sample halves per fp6 unpack workgroup: arrays under 1024 elements
 // take several, so that all 1024 element slots work
@@ -179,22 +178,25 @@ __host__ __device__ inline int64_t f6_u4_units(int64_t KB, int Np) {
   const int hpu = fp6_halves_per_unit(Np);
   return (2 * KB + hpu - 1) / hpu;
 }
-// Runs (workgroups per channel and element group) of an FP6 batch unpack
-// job: a run takes units/8 units, at least 1 and at most 16 (so that its
-// code sums, at most 15 * 32 * 8 * 16 per element, fit 16 bits); a small
-// job (a streamed frame: 32 units) still spreads over many workgroups.
-__host__ __device__ inline int f6_u4_job_runs(int64_t KB, int Np) {
-  const int64_t units = f6_u4_units(KB, Np);
+// Unpack units of a chunk of KB layout blocks (per channel and element
+// group): int8 one per 16-sample block, FP6 f6_u4_units.
+__host__ __device__ inline int64_t u4_units(int fmt, int64_t KB, int Np) {
+  return fmt == kPlanesFp6 ? f6_u4_units(KB, Np) : KB;
+}
+// Runs (workgroups per channel and element group) of a batch unpack job: a
+// run takes units/8 units, at least 1 and at most 16 (so that its code
+// sums, at most 15 * 32 * 8 * 16 per element, fit 16 bits); a small job (a
+// streamed frame: 32 FP6 units) still spreads over many workgroups.
+__host__ __device__ inline int u4_job_runs(int fmt, int64_t KB, int Np) {
+  const int64_t units = u4_units(fmt, KB, Np);
   int64_t per = units / 8;
   per = per < 1 ? 1 : (per > 16 ? 16 : per);
   return (int)((units + per - 1) / per);
 }
-inline int64_t unpack_units(int fmt, const UnpackJob &u) {
-  const int64_t groups = (u.Np + 1023) / 1024;
-  if (fmt != kPlanesFp6) return groups * u.KB * u.F;
-  return groups * u.runs * u.F;
+inline int64_t unpack_units(int, const UnpackJob &u) {
+  return (u.Np + 1023) / 1024 * u.runs * u.F;
 }
-// The offset corrections of one GEMM job (FP6 batch and serve): partials
+// The offset corrections of one GEMM job (batch and serve): partials
 // [nrun][F][Np] of code sums over a chunk of n valid samples -> corr [2][F]
 // [Np] int32 pairs (rows, then columns), as launch_u4_corrections; the
 // job's workgroups start at blk0 (64 elements each, increasing over jobs).

@@ -11,39 +11,32 @@
 // There is no time_accum cap.
 //
 // Two passes per chunk of time:
-//  1. unpack (HBM-bound): [time][freq][element] bytes -> MFMA-ready planes
-//     [plane][freq][16-sample block][element][16 B], elements past N and
-//     samples past T zero.  Two encodings of a nibble u (value u - 8):
-//     * plan path (unpack_u4_kernel): the offset-binary code u itself, in
-//       [0, 15].  The GEMM then accumulates code products, and -Br becomes
-//       the complement 15 - u (one XOR):
-//         RE' = sum ur_a ur_b + ui_a ui_b,  IM' = sum ur_a ui_b + ui_a (15 - ur_b),
-//       which the per-element code sums SR, SI over the chunk's n valid
-//       samples turn into the signed values (padding is code 0 and adds
-//       nothing):
-//         RE = RE' - 8 (SR_a + SI_a) + 128 n - 8 (SR_b + SI_b)
-//         IM = IM' - 8 SR_a - 7 SI_a + 8 (SR_b - SI_b).
-//       The unpack writes each workgroup's code sums, a small kernel adds
-//       them into one row and one column correction per element
-//       (u4_corrections_kernel), and the GEMM adds them as it flushes.  Codes
-//       switch four bits per operand byte and no sign bits, so the
-//       power-limited matrix cores hold a higher clock than with signed
-//       operands.
-//     * batch and serve (unpack_unit): the byte XORed with 0x88, each
-//       nibble's 4-bit two's complement placed in bits 6..3 of an int8 with
-//       its sign in bit 7: 8*value exactly; products are 64x the true ones,
-//       exact with no correction.
+//  1. unpack (HBM-bound; unpack_u4_kernel for a plan, unpack_u4_batch_kernel
+//     for the batch and serve engines): [time][freq][element] bytes ->
+//     MFMA-ready planes [plane][freq][16-sample block][element][16 B] of the
+//     nibbles' offset-binary codes u in [0, 15] (value u - 8), elements past
+//     N and samples past T code 0.  The GEMM then accumulates code products,
+//     and -Br becomes the complement 15 - u (one XOR):
+//       RE' = sum ur_a ur_b + ui_a ui_b,  IM' = sum ur_a ui_b + ui_a (15 - ur_b),
+//     which the per-element code sums SR, SI over the chunk's n valid
+//     samples turn into the signed values (padding adds nothing):
+//       RE = RE' - 8 (SR_a + SI_a) + 128 n - 8 (SR_b + SI_b)
+//       IM = IM' - 8 SR_a - 7 SI_a + 8 (SR_b - SI_b).
+//     The unpack writes each workgroup's code sums, a small kernel adds them
+//     into one row and one column correction per element
+//     (u4_corrections_kernel, u4_corrections_batch_kernel), and the GEMM
+//     adds them as it flushes.  Codes switch four bits per operand byte and
+//     no sign bits, and every product is >= 0, so the power-limited matrix
+//     cores hold a higher clock than with signed operands.
 //  2. corr_mfma_kernel / corr_mfma_batch_kernel: a workgroup = 4 wavefronts
 //     = a 128-row x 256-column tile of one channel, each wave all 128 rows
-//     x 64 columns (4x2 blocks x 2 int32 accumulators: RE = Ar.Br + Ai.Bi
-//     and IM = Ar.Bi + Ai.(-Br), -Br formed per byte in registers).  The
+//     x 64 columns (4x2 blocks x 2 int32 accumulators: RE' and IM').  The
 //     row operands come through a ring of 4 LDS k-steps of 32 samples (8 KB
 //     each) filled by buffer->LDS DMA; each wave loads its own column
 //     operands straight into VGPRs, three k-steps ahead.
 // A work item covers at most kMaxItemStages 64-sample stages (131072
-// samples): signed products are multiples of 64 of magnitude <= 2^30 per
-// accumulator, divided by 64 (exact) when flushed; code products add at most
-// 450 per sample (< 2^26 per accumulator).
+// samples): code products add at most 450 per sample, < 2^26 per
+// accumulator.
 #include "chx_internal.h"
 
 namespace chx {
@@ -82,13 +75,6 @@ __device__ __forceinline__ void transpose4(uint32_t r0, uint32_t r1, uint32_t r2
   c3 = __builtin_amdgcn_perm(t23h, t01h, 0x07060302u);
 }
 
-// Real and imaginary planes of four packed samples, each value times 8.
-__device__ __forceinline__ void planes(uint32_t w, uint32_t &re, uint32_t &im) {
-  const uint32_t t = w ^ 0x88888888u;  // per byte: R two's complement in 7..4, I in 3..0
-  re = ((t >> 1) & 0x78787878u) | (t & 0x80808080u);         // 8 R, per byte
-  im = ((t << 3) & 0x78787878u) | ((t << 4) & 0x80808080u);  // 8 I, per byte
-}
-
 // Maps the dispatch-order workgroup id so that the workgroups resident on
 // one XCD (dispatch is round-robin over the 8 XCDs) take consecutive
 // (channel, tile) work items and share their input rows in that XCD's L2.
@@ -98,42 +84,14 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t id, int64_t total) {
   return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
 }
 
-// Flush one block's accumulators.  Lane l owns column n = l%32 and rows
-// m = 8*(j/4) + 4*(l/32) + j%4 of register j (v_mfma_i32_32x32x32_i8 D
-// layout).  re holds 64 (Ar.Br + Ai.Bi), im 64 (Ar.Bi - Ai.Br).  mode 0
-// assigns, 1 adds, 2 adds atomically.
-__device__ __forceinline__ void flush_block(int32_t *blk, const v16i &re, const v16i &im, int U,
-                                            int mode, int lane) {
-  // Buffer ops on a descriptor of the block (a wave-uniform base): one VGPR
-  // of lane offset, the register's row as a constant scalar offset.
-  const __amdgpu_buffer_rsrc_t rsrc =
-      __builtin_amdgcn_make_buffer_rsrc(blk, (short)0, 2048 * 4, 0x00020000);
-  const int voff = ((lane >> 5) * 4 * 32 + (lane & 31)) * 8;
-#pragma unroll
-  for (int j = 0; j < 16; j++) {
-    const int32_t vr = re[j] >> 6;  // exact: every product is a multiple of 64
-    const int32_t vi = U ? -(im[j] >> 6) : im[j] >> 6;
-    const int soff = (8 * (j >> 2) + (j & 3)) * 32 * 8;  // row m = 8(j/4) + 4(l/32) + j%4
-    if (mode == 0) {
-      v2i v = {vr, vi};
-      __builtin_amdgcn_raw_buffer_store_b64(v, rsrc, voff, soff, 0);
-    } else if (mode == 1) {
-      v2i a = __builtin_amdgcn_raw_buffer_load_b64(rsrc, voff, soff, 0);
-      a.x += vr;
-      a.y += vi;
-      __builtin_amdgcn_raw_buffer_store_b64(a, rsrc, voff, soff, 0);
-    } else {
-      __builtin_amdgcn_raw_ptr_buffer_atomic_add_i32(vr, rsrc, voff, soff, 0);
-      __builtin_amdgcn_raw_ptr_buffer_atomic_add_i32(vi, rsrc, voff + 4, soff, 0);
-    }
-  }
-}
-
-// Flush one block of code-product accumulators (re = RE', im = IM'), adding
-// the offset corrections: rc[j] = (row RE, row IM) of register j's row, cc
-// = (column RE, column IM) of the lane's column (zeros for a piece that
-// does not start the chunk).  Same layout and modes as flush_block.
-__device__ __forceinline__ void flush_block_u4(int32_t *blk, const v16i &re, const v16i &im,
+// Flush one block of code-product accumulators (re = RE', im = IM') into
+// int32 visibilities, adding the offset corrections: rc[j] = (row RE, row
+// IM) of register j's row, cc = (column RE, column IM) of the lane's column
+// (zeros for a piece that does not start the chunk).  Lane l owns column n
+// = l%32 and rows m = 8*(j/4) + 4*(l/32) + j%4 of register j
+// (v_mfma_i32_32x32x32_i8 D layout).  mode 0 assigns, 1 adds, 2 adds
+// atomically.
+__device__ __forceinline__ void flush_block(int32_t *blk, const v16i &re, const v16i &im,
                                                int U, int mode, int lane, const v2i (&rc)[16],
                                                const v2i cc) {
   const __amdgpu_buffer_rsrc_t rsrc =
@@ -160,96 +118,34 @@ __device__ __forceinline__ void flush_block_u4(int32_t *blk, const v16i &re, con
   }
 }
 
-// Pass 1.  Workgroup = 256 element quads (1024 elements) x 16 samples of
-// one channel.  Thread: 16 word loads (one per sample row; 256 contiguous
-// bytes per wave and row), four 4x4 byte transposes, plane split; the
-// planes go through LDS so that every global store writes 1 KB of
-// consecutive 16-byte element chunks (full cache lines).
-// KB is the destination's block count per channel (its layout), kb the
-// destination block written (kb - kb_src0 the source's 16-sample block).
-__device__ __forceinline__ void unpack_unit(const uint8_t *__restrict__ in,
-                                            uint8_t *__restrict__ out_planes, int N, int F,
-                                            int Np, int64_t row_bytes, int64_t t0,
-                                            int64_t t_valid, int KB, int qb, int kb,
-                                            int kb_src0, int f, v4u (*stage)[1024 + 64]) {
-  const int tid = threadIdx.x;
-  const int q = qb * 256 + tid;  // element quad
-  const bool evalid = 4 * q < N;
-  uint32_t w[16];
-  const int64_t ks = kb - kb_src0;  // source block
-  const uint8_t *src = in + (t0 + 16 * ks) * row_bytes + (int64_t)f * N + 4 * q;
-#pragma unroll
-  for (int s = 0; s < 16; s++) {
-    const bool ok = evalid && (16 * ks + s < t_valid);
-    // streamed once: non-temporal loads (and stores below)
-    w[s] = ok ? __builtin_nontemporal_load((const uint32_t *)(src + s * row_bytes)) : 0x88888888u;
-  }
-  uint32_t c[4][4];  // [element][4-sample group]
-#pragma unroll
-  for (int m = 0; m < 4; m++)
-    transpose4(w[4 * m], w[4 * m + 1], w[4 * m + 2], w[4 * m + 3], c[0][m], c[1][m], c[2][m],
-               c[3][m]);
-#pragma unroll
-  for (int e = 0; e < 4; e++) {
-    uint32_t r[4], i[4];
-#pragma unroll
-    for (int m = 0; m < 4; m++) planes(c[e][m], r[m], i[m]);
-    const int el = 4 * tid + e;
-    const int slot = el + (el >> 4);  // one pad chunk per 16: conflict-free b128 writes
-    stage[0][slot] = (v4u){r[0], r[1], r[2], r[3]};
-    stage[1][slot] = (v4u){i[0], i[1], i[2], i[3]};
-  }
-  __syncthreads();
-  const int64_t plane_stride = (int64_t)F * KB * Np * 16;
-  const int e_base = qb * 1024;
-  uint8_t *dst = out_planes + (((int64_t)f * KB + kb) * Np + e_base) * 16;
-  const int n_el = Np - e_base < 1024 ? Np - e_base : 1024;
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const int el = k * 256 + tid;
-    if (el < n_el) {
-      const int slot = el + (el >> 4);
-      __builtin_nontemporal_store(stage[0][slot], (v4u *)(dst + el * 16));
-      __builtin_nontemporal_store(stage[1][slot], (v4u *)(dst + plane_stride + el * 16));
-    }
-  }
-}
-
-// Pass 1 of the plan path: offset-binary code planes.  Workgroup = 256
-// element quads (1024 elements) of one channel x the 16-sample blocks run,
-// run + R, run + 2R, ... of the chunk (R = gridDim.z runs); grid x =
-// channel, y = element group, z = run.  About kU4Workgroups workgroups in
-// all, so that at any time the resident ones unpack neighbouring blocks of
-// every channel, as a grid of one block per workgroup would (workgroups
-// running together read whole input rows and write neighbouring plane
-// rows).  Per block, as unpack_unit: 16 word loads per thread (the next
-// block's issued before this one is written), four 4x4 byte transposes,
-// the codes split into the R and I planes, LDS staging, full-run stores.
+// Pass 1: offset-binary code planes.  One workgroup = one run of 256
+// element quads (1024 elements) of channel f, element group qb: the
+// 16-sample blocks run, run + R, run + 2R, ... of the chunk, so that at any
+// time the resident workgroups unpack neighbouring blocks of every channel,
+// as a grid of one block per workgroup would (workgroups running together
+// read whole input rows and write neighbouring plane rows).  Per block: 16
+// word loads per thread (one per sample row; 256 contiguous bytes per wave
+// and row; the next block's issued before this one is written), four 4x4
+// byte transposes, the codes split into the R and I planes, LDS staging so
+// that every global store writes 1 KB of consecutive 16-byte element chunks
+// (full cache lines).  Loads and stores go through buffer descriptors,
+// branch-free: a block's descriptor covers its valid sample rows only, and
+// lanes past N (or past the block's last element) use an offset beyond
+// every descriptor, so they read zeros and their stores are dropped;
+// streamed once: non-temporal (aux 2).  Source block kb (relative to t0)
+// lands at destination block kb_dst0 + kb of a layout with KB_dst blocks.
 // Each thread also adds up its four elements' R and I codes over its
 // blocks and stores them as one word per element (SR | SI << 16; at most
-// 16 * 15 * kU4MaxBlocks = 61440) into partials[run][f][element].  Samples
-// past t_valid and elements past N are code 0.
-constexpr int kU4Workgroups = 1024;
-constexpr int kU4MaxBlocks = 256;  // blocks per workgroup: the partials fit 16 bits
-__host__ __device__ inline int64_t u4_runs(int64_t KB, int64_t F, int64_t Np) {
-  const int64_t per = F * ((Np + 1023) / 1024);
-  int64_t r = (kU4Workgroups + per - 1) / per;
-  const int64_t need = (KB + kU4MaxBlocks - 1) / kU4MaxBlocks;
-  if (r < need) r = need;
-  return r < KB ? r : KB;
-}
-__global__ __launch_bounds__(256) void unpack_u4_kernel(
-    const uint8_t *__restrict__ in, uint8_t *__restrict__ out_planes,
-    uint32_t *__restrict__ partials, int N, int F, int Np, int64_t row_bytes, int64_t t0,
-    int64_t t_valid, int KB) {
-  __shared__ __attribute__((aligned(16))) v4u stage[2][1024 + 64];  // [plane][element], padded
-  const int f = blockIdx.x, qb = blockIdx.y, run = blockIdx.z, R = gridDim.z;
+// 16 * 15 * 256 = 61440 with at most 256 blocks per run) into prow[f]
+// [element].  Samples past t_valid and elements past N are code 0.
+__device__ __forceinline__ void unpack_u4_run(const uint8_t *__restrict__ in,
+                                              uint8_t *__restrict__ out_planes,
+                                              uint32_t *__restrict__ prow, int N, int F, int Np,
+                                              int64_t row_bytes, int64_t t0, int64_t t_valid,
+                                              int KB, int KB_dst, int kb_dst0, int f, int qb,
+                                              int run, int R, v4u (*stage)[1024 + 64]) {
   const int tid = threadIdx.x;
   const int q = qb * 256 + tid;  // element quad
-  // Buffer loads and stores, branch-free: a block's descriptor covers its
-  // valid sample rows only, and lanes past N (or past the block's last
-  // element) use an offset beyond every descriptor, so they read zeros
-  // and their stores are dropped.  Streamed once: non-temporal (aux 2).
   const int rb = (int)row_bytes;  // 16 rows of a block fit in an int
   // lanes that read nothing: an offset past every descriptor (the row
   // offsets added below keep it there, modulo 2^32)
@@ -263,7 +159,7 @@ __global__ __launch_bounds__(256) void unpack_u4_kernel(
     for (int s = 0; s < 16; s++)
       w[s] = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)(voff_in + (uint32_t)(s * rb)), 0, 2);
   };
-  const int64_t plane_stride = (int64_t)F * KB * Np * 16;
+  const int64_t plane_stride = (int64_t)F * KB_dst * Np * 16;
   const int e_base = qb * 1024;
   const int n_el = Np - e_base < 1024 ? Np - e_base : 1024;
   uint32_t sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
@@ -293,7 +189,7 @@ __global__ __launch_bounds__(256) void unpack_u4_kernel(
       stage[1][slot] = (v4u){i[0], i[1], i[2], i[3]};
     }
     __syncthreads();
-    uint8_t *dst = out_planes + (((int64_t)f * KB + kb) * Np + e_base) * 16;
+    uint8_t *dst = out_planes + (((int64_t)f * KB_dst + kb_dst0 + kb) * Np + e_base) * 16;
     const __amdgpu_buffer_rsrc_t d0 =
         __builtin_amdgcn_make_buffer_rsrc(dst, (short)0, n_el * 16, 0x00020000);
     const __amdgpu_buffer_rsrc_t d1 =
@@ -310,8 +206,54 @@ __global__ __launch_bounds__(256) void unpack_u4_kernel(
   if (4 * q < Np) {
     const v4u v = {sr[0] | si[0] << 16, sr[1] | si[1] << 16, sr[2] | si[2] << 16,
                    sr[3] | si[3] << 16};
-    *(v4u *)(partials + ((int64_t)run * F + f) * Np + 4 * q) = v;
+    *(v4u *)(prow + (int64_t)f * Np + 4 * q) = v;
   }
+}
+
+// Plan chunk: grid x = channel, y = element group, z = run (about
+// kU4Workgroups workgroups in all); run r writes partials row r.
+constexpr int kU4Workgroups = 1024;
+constexpr int kU4MaxBlocks = 256;  // blocks per workgroup: the partials fit 16 bits
+__host__ __device__ inline int64_t u4_runs(int64_t KB, int64_t F, int64_t Np) {
+  const int64_t per = F * ((Np + 1023) / 1024);
+  int64_t r = (kU4Workgroups + per - 1) / per;
+  const int64_t need = (KB + kU4MaxBlocks - 1) / kU4MaxBlocks;
+  if (r < need) r = need;
+  return r < KB ? r : KB;
+}
+__global__ __launch_bounds__(256) void unpack_u4_kernel(
+    const uint8_t *__restrict__ in, uint8_t *__restrict__ out_planes,
+    uint32_t *__restrict__ partials, int N, int F, int Np, int64_t row_bytes, int64_t t0,
+    int64_t t_valid, int KB) {
+  __shared__ __attribute__((aligned(16))) v4u stage[2][1024 + 64];  // [plane][element], padded
+  unpack_u4_run(in, out_planes, partials + (int64_t)blockIdx.z * F * Np, N, F, Np, row_bytes, t0,
+                t_valid, KB, KB, 0, blockIdx.x, blockIdx.y, blockIdx.z, gridDim.z, stage);
+}
+
+// Batch: one launch unpacks the chunks of many jobs.  Workgroup g belongs
+// to the job j with jobs[j].unit0 <= g < jobs[j+1].unit0 (binary search);
+// its index within the job is (run, element group, channel), channel
+// fastest so that neighbours read whole input rows; run r writes the job's
+// partial row prow0 + r.
+__global__ __launch_bounds__(256) void unpack_u4_batch_kernel(const UnpackJob *__restrict__ jobs,
+                                                              int njobs) {
+  __shared__ __attribute__((aligned(16))) v4u stage[2][1024 + 64];
+  const int64_t g = blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].unit0 <= g) lo = mid; else hi = mid - 1;
+  }
+  const UnpackJob &J = jobs[lo];
+  const int64_t u = g - J.unit0;
+  const int nq = (J.Np + 1023) / 1024;
+  if (u >= (int64_t)nq * J.runs * J.F) return;  // grid padding (uniform per workgroup)
+  const int f = (int)(u % J.F);
+  const int qb = (int)((u / J.F) % nq);
+  const int run = (int)(u / ((int64_t)J.F * nq));
+  unpack_u4_run(J.in, J.planes, J.partials + (int64_t)(J.prow0 + run) * J.F * J.Np, J.N, J.F,
+                J.Np, J.row_bytes, J.t0, J.t_valid, J.KB, J.KB_dst, J.kb_dst0, f, qb, run, J.runs,
+                stage);
 }
 
 // The offset corrections of one chunk (n valid samples): per channel f and
@@ -370,32 +312,6 @@ __global__ __launch_bounds__(256) void u4_corrections_batch_kernel(
   u4_corrections_block(J.partials, J.corr, J.F, J.Np, J.nrun, J.n, blk);
 }
 
-// Batch: one launch unpacks the chunks of many jobs.  Workgroup g belongs
-// to the job j with jobs[j].unit0 <= g < jobs[j+1].unit0 (binary search);
-// its unit index within the job is (block, element group, channel), channel
-// fastest so that neighbours read whole input rows.  A job
-// writes blocks [kb_dst0, kb_dst0 + KB) of a destination laid out with
-// KB_dst blocks per channel (several frames of one stream side by side).
-__global__ __launch_bounds__(256) void unpack_batch_kernel(const UnpackJob *__restrict__ jobs,
-                                                           int njobs) {
-  __shared__ __attribute__((aligned(16))) v4u stage[2][1024 + 64];
-  const int64_t g = blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
-  int lo = 0, hi = njobs - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) / 2;
-    if (jobs[mid].unit0 <= g) lo = mid; else hi = mid - 1;
-  }
-  const UnpackJob &J = jobs[lo];
-  const int64_t u = g - J.unit0;
-  const int nq = (J.Np + 1023) / 1024;
-  if (u >= (int64_t)nq * J.KB * J.F) return;  // grid padding (uniform per workgroup)
-  const int f = (int)(u % J.F);
-  const int qb = (int)((u / J.F) % nq);
-  const int kb = (int)(u / ((int64_t)J.F * nq));
-  unpack_unit(J.in, J.planes, J.N, J.F, J.Np, J.row_bytes, J.t0, J.t_valid, J.KB_dst, qb,
-              J.kb_dst0 + kb, J.kb_dst0, f, stage);
-}
-
 // Tile tb of one channel.  First the full 128 x 256 tiles, row by row: row
 // tile ti holds floor((ntiles - ti) / 2) of them, columns from kTile ct, ct
 // = ti, ti + 2, ...  Then the half tiles, the last tile of each row tile ti
@@ -428,15 +344,14 @@ __device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti_a, int &
 //          tile; the four waves share the rows);
 //    PAIR: two half tiles of column tile ct: waves 0-1 rows from kTile
 //          ti_a, waves 2-3 rows from kTile ti_b, columns kTile ct + 64 (w%2);
-// RE += Ar.Br + Ai.Bi,  IM += Ar.Bi + Ai.(-Br),  where -Br is formed per
-// byte in registers: !U4 (8 x value bytes, in [-64, 56]) the exact
-// negation; U4 (offset-binary codes, in [0, 15]) the complement 15 - u,
-// and the flush adds the offset corrections J.corr.  The
+// RE' += Ar.Br + Ai.Bi,  IM' += Ar.Bi + Ai.(15 - Br) on offset-binary codes
+// (bytes in [0, 15]; 15 - u per byte in registers, one XOR per word), and
+// the flush adds the offset corrections J.corr.  The
 // row operands, shared by the four waves, come through the LDS ring by
 // buffer->LDS DMA; each wave loads its own column operands straight into
 // VGPRs, three k-steps ahead, so every operand byte crosses the CU's load
 // path once.
-template <bool PAIR, bool U4 = false>
+template <bool PAIR>
 __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const int ti_a,
                                           const int ti_b, const int ct, const int st_begin,
                                           const int nst, uint8_t (*lds)[kSlotBytes]) {
@@ -521,16 +436,8 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
   auto step_ops = [&](const v4i &br0, const v4i &br1, const v4i &bi0, const v4i &bi1,
                       int nslot) {
     const v4i br[2] = {br0, br1}, bi[2] = {bi0, bi1};
-    // -Br per byte: IM += Ar.Bi + Ai.(-Br).  Signed bytes (multiples of 8)
-    // are negated exactly; codes u are complemented to 15 - u.
-    v4i nbr[2];
-    if constexpr (U4) {
-      nbr[0] = br0 ^ 0x0F0F0F0F;
-      nbr[1] = br1 ^ 0x0F0F0F0F;
-    } else {
-      nbr[0] = ((br0 ^ (int)0xF8F8F8F8u) + 0x08080808) & (int)0xF8F8F8F8u;
-      nbr[1] = ((br1 ^ (int)0xF8F8F8F8u) + 0x08080808) & (int)0xF8F8F8F8u;
-    }
+    // 15 - Br per byte: IM' += Ar.Bi + Ai.(15 - Br)
+    const v4i nbr[2] = {br0 ^ 0x0F0F0F0F, br1 ^ 0x0F0F0F0F};
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const v4i xr = i == 3 ? ar3n : ar[i], xi = i == 3 ? ai3n : ai[i];
@@ -636,57 +543,43 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
       __builtin_amdgcn_sched_barrier(0);
     }
   }
-  // Signed operands: every partial sum is a multiple of 64 of magnitude <=
-  // 2 * 64 * 64 * 32 * (2 * kMaxItemStages) = 2^30.  Codes: <= 450 per
-  // sample, < 2^26.  No int32 overflow either way.
-  if constexpr (U4) {
-    // Corrections: the piece that starts the chunk adds them, once.
-    const bool first = st_begin == 0;
-    const v2i *rowc = (const v2i *)J.corr + (int64_t)f * Np;
-    const v2i *colc = (const v2i *)J.corr + ((int64_t)F + f) * Np;
-    v2i cc[2];
+  // Code products add at most 450 per sample: < 2^26 per item, no int32
+  // overflow.
+  // Corrections: the piece that starts the chunk adds them, once.
+  const bool first = st_begin == 0;
+  const v2i *rowc = (const v2i *)J.corr + (int64_t)f * Np;
+  const v2i *colc = (const v2i *)J.corr + ((int64_t)F + f) * Np;
+  v2i cc[2];
 #pragma unroll
-    for (int j = 0; j < 2; j++)
-      cc[j] = first ? colc[(bx0 + j) * 32 + (lane & 31)] : (v2i){0, 0};
+  for (int j = 0; j < 2; j++)
+    cc[j] = first ? colc[(bx0 + j) * 32 + (lane & 31)] : (v2i){0, 0};
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int by = by0 + i;
-      // rows of register j: 32 by + 8 (j/4) + 4 (lane/32) + j%4
-      v2i rc[16];
+  for (int i = 0; i < 4; i++) {
+    const int by = by0 + i;
+    // rows of register j: 32 by + 8 (j/4) + 4 (lane/32) + j%4
+    v2i rc[16];
 #pragma unroll
-      for (int g = 0; g < 4; g++) {
-        v4i lo = {0, 0, 0, 0}, hi = {0, 0, 0, 0};
-        if (first) {
-          const v2i *p = rowc + by * 32 + 8 * g + 4 * (lane >> 5);
-          lo = *(const v4i *)p;
-          hi = *(const v4i *)(p + 2);
-        }
-        rc[4 * g] = (v2i){lo.x, lo.y};
-        rc[4 * g + 1] = (v2i){lo.z, lo.w};
-        rc[4 * g + 2] = (v2i){hi.x, hi.y};
-        rc[4 * g + 3] = (v2i){hi.z, hi.w};
+    for (int g = 0; g < 4; g++) {
+      v4i lo = {0, 0, 0, 0}, hi = {0, 0, 0, 0};
+      if (first) {
+        const v2i *p = rowc + by * 32 + 8 * g + 4 * (lane >> 5);
+        lo = *(const v4i *)p;
+        hi = *(const v4i *)(p + 2);
       }
+      rc[4 * g] = (v2i){lo.x, lo.y};
+      rc[4 * g + 1] = (v2i){lo.z, lo.w};
+      rc[4 * g + 2] = (v2i){hi.x, hi.y};
+      rc[4 * g + 3] = (v2i){hi.z, hi.w};
+    }
 #pragma unroll
-      for (int j = 0; j < 2; j++) {
-        const int bx = bx0 + j;
-        if (bx >= by && bx < nb) {
-          const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
-          flush_block_u4(vis + ((int64_t)f * NB + b) * 2048, re[i][j], im[i][j], U, mode, lane,
-                         rc, cc[j]);
-        }
+    for (int j = 0; j < 2; j++) {
+      const int bx = bx0 + j;
+      if (bx >= by && bx < nb) {
+        const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
+        flush_block(vis + ((int64_t)f * NB + b) * 2048, re[i][j], im[i][j], U, mode, lane,
+                       rc, cc[j]);
       }
     }
-  } else {
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-      for (int j = 0; j < 2; j++) {
-        const int by = by0 + i, bx = bx0 + j;
-        if (bx >= by && bx < nb) {
-          const int64_t b = (int64_t)by * nb - (int64_t)by * (by - 1) / 2 + (bx - by);
-          flush_block(vis + ((int64_t)f * NB + b) * 2048, re[i][j], im[i][j], U, mode, lane);
-        }
-      }
   }
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
@@ -716,9 +609,9 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(GemmJob J, int nst, i
   int ti_a, ti_b, ct;
   tile_coords((int)(item % per_f), J.ntiles, ti_a, ti_b, ct);
   if (ti_b >= 0 && ti_b != ti_a)
-    gemm_tile<true, true>(J, (int)(item / per_f), ti_a, ti_b, ct, s0, ns, lds);
+    gemm_tile<true>(J, (int)(item / per_f), ti_a, ti_b, ct, s0, ns, lds);
   else
-    gemm_tile<false, true>(J, (int)(item / per_f), ti_a, ti_a, ct, s0, ns, lds);
+    gemm_tile<false>(J, (int)(item / per_f), ti_a, ti_a, ct, s0, ns, lds);
 }
 
 // Batch: persistent workgroups take work items from 8 per-XCD queues: a
@@ -916,7 +809,7 @@ int launch_unpack_batch_int8(const UnpackJob *d_jobs, int njobs, int64_t units, 
   if (units <= 0) return CHX_SUCCESS;
   const int64_t gx = units < 65536 ? units : 65536;
   const int64_t gy = (units + gx - 1) / gx;
-  unpack_batch_kernel<<<dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s>>>(d_jobs, njobs);
+  unpack_u4_batch_kernel<<<dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s>>>(d_jobs, njobs);
   CHX_HIP(hipGetLastError());
   return CHX_SUCCESS;
 }

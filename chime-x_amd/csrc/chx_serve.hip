@@ -77,22 +77,21 @@ int64_t group_stages(int fmt, const Stream &s, int64_t k) {
 int64_t group_plane_bytes(int fmt, const Stream &s, int64_t k) {
   return chx::format_plane_bytes(fmt, s.F, s.Np, group_stages(fmt, s, k));
 }
-// FP6 code-sum rows of k frames side by side: each frame's unpack runs,
-// then the ring padding's.
-int64_t group_rows(const Stream &s, int64_t k) {
-  const int64_t pad = group_blocks(chx::kPlanesFp6, s, k) - k * s.KBf;
-  return k * chx::f6_u4_job_runs(s.KBf, (int)s.Np) +
-         (pad > 0 ? chx::f6_u4_job_runs(pad, (int)s.Np) : 0);
+// Code-sum rows of k frames side by side: each frame's unpack runs, then
+// the ring padding's (FP6).
+int64_t group_rows(int fmt, const Stream &s, int64_t k) {
+  const int64_t pad = group_blocks(fmt, s, k) - k * s.KBf;
+  return k * chx::u4_job_runs(fmt, s.KBf, (int)s.Np) +
+         (pad > 0 ? chx::u4_job_runs(fmt, pad, (int)s.Np) : 0);
 }
-int64_t group_partial_bytes(const Stream &s, int64_t k) {
-  return round_up(group_rows(s, k) * s.F * s.Np * 4, 256);
+int64_t group_partial_bytes(int fmt, const Stream &s, int64_t k) {
+  return round_up(group_rows(fmt, s, k) * s.F * s.Np * 4, 256);
 }
-// Workspace of k frames of a stream: planes, and (FP6) code-sum rows and
-// offset corrections.
+// Workspace of k frames of a stream: planes, code-sum rows and offset
+// corrections.
 int64_t group_bytes(int fmt, const Stream &s, int64_t k) {
-  int64_t b = group_plane_bytes(fmt, s, k);
-  if (fmt == chx::kPlanesFp6) b += group_partial_bytes(s, k) + round_up(2 * s.F * s.Np * 8, 256);
-  return b;
+  return group_plane_bytes(fmt, s, k) + group_partial_bytes(fmt, s, k) +
+         round_up(2 * s.F * s.Np * 8, 256);
 }
 // One (array, plane, channel) region of a group must stay addressable by a
 // 32-bit buffer descriptor.
@@ -165,11 +164,10 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
     const int KBt = (int)group_blocks(fmt, s, (int64_t)m.size());
     uint8_t *planes = S->d_work + off;
     uint8_t *planes1 = fmt == kPlanesFp6 ? planes + 64LL * s.F * KBt * s.Np : nullptr;
-    const bool codes = fmt == kPlanesFp6;  // FP6: offset-binary codes and corrections
     const int64_t pbytes = group_plane_bytes(fmt, s, (int64_t)m.size());
-    uint32_t *partials = codes ? (uint32_t *)(planes + pbytes) : nullptr;
+    uint32_t *partials = (uint32_t *)(planes + pbytes);
     int32_t *corr =
-        codes ? (int32_t *)(planes + pbytes + group_partial_bytes(s, (int64_t)m.size())) : nullptr;
+        (int32_t *)(planes + pbytes + group_partial_bytes(fmt, s, (int64_t)m.size()));
     off += group_bytes(fmt, s, (int64_t)m.size());
     int prow = 0;
     for (int k = 0; k < (int)m.size(); k++) {
@@ -189,7 +187,7 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
       u.kb_dst0 = k * s.KBf;
       u.partials = partials;
       u.prow0 = prow;
-      u.runs = codes ? f6_u4_job_runs(u.KB, (int)s.Np) : 0;
+      u.runs = u4_job_runs(fmt, u.KB, (int)s.Np);
       prow += u.runs;
       ujobs.push_back(u);
       units += unpack_units(fmt, u);
@@ -202,13 +200,13 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
       z.KB = KBt - used;
       z.kb_dst0 = used;
       z.prow0 = prow;
-      z.runs = codes ? f6_u4_job_runs(z.KB, (int)s.Np) : 0;
+      z.runs = u4_job_runs(fmt, z.KB, (int)s.Np);
       prow += z.runs;
       ujobs.push_back(z);
       units += unpack_units(fmt, z);
     }
-    if (codes) {
-      if (prow != group_rows(s, (int64_t)m.size())) return CHX_ERR_HIP;
+    {
+      if (prow != group_rows(fmt, s, (int64_t)m.size())) return CHX_ERR_HIP;
       CorrJob c;
       c.partials = partials;
       c.corr = corr;
@@ -279,7 +277,7 @@ int run_launch(chx_serve *S, const std::vector<Frame> &batch,
   if (!rc)
     rc = launch_unpack_batch(S->fmt, (const UnpackJob *)(S->d_tab + a0), (int)ujobs.size(),
                              units, cs);
-  if (!rc && !cjobs.empty())
+  if (!rc)
     rc = launch_u4_corrections_batch((const CorrJob *)(S->d_tab + ac), (int)cjobs.size(), cblocks,
                                      cs);
   if (!rc)

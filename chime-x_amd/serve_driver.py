@@ -92,6 +92,8 @@ def main() -> int:
     ap.add_argument("--max-item-stages", type=int, default=0,
                     help="cut GEMM work items into pieces of this many 64-sample stages (0 = whole)")
     ap.add_argument("--modes", default="device_resident,host_pcie")
+    ap.add_argument("--method", choices=["fp6", "int8"], default="fp6",
+                    help="matrix-core operands (bit-identical results)")
     ap.add_argument("--out", default="")
     a = ap.parse_args()
 
@@ -115,8 +117,10 @@ def main() -> int:
     loads = [float(x) for x in a.loads.split(",")]
     res = dict(workload=f"streaming: N={N}, F={F}, frames of {T} samples, {n} frames "
                         f"({n * T} samples), 5% of elements held at (7,-8)", runs={})
+    method = chx.FP6_MFMA if a.method == "fp6" else chx.INT8_MFMA
+    res["method"] = a.method
     with chx.Serve(frame_slots=a.slots, slot_bytes=fb,
-                   max_item_stages=a.max_item_stages) as sv:
+                   max_item_stages=a.max_item_stages, method=method) as sv:
         for mode, frames, device in (("device_resident", dev_frames, True),
                                      ("host_pcie", host_frames, False)):
             if mode not in a.modes.split(","):

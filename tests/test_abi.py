@@ -309,6 +309,32 @@ def test_small_tiles_cover_the_upper_triangle_once():
         assert len(seen) == 4 * nt * (4 * nt + 1) // 2
 
 
+def _batch_workspace(jobs, chunk, unit, method):
+    """Workspace of a batch plan at a given chunk: every job's planes, then
+    per job the most code-sum rows (runs per channel and element group) any
+    round writes, 4 bytes per element, and two 8-byte corrections per
+    element, each region 256-aligned."""
+    r256 = lambda x: -(-x // 256) * 256
+    ws = 0
+    nps = [-(-j["num_elements"] // 128) * 128 for j in jobs]
+    for j, Np in zip(jobs, nps):
+        stages = min(chunk, -(-j["time_steps"] // unit) * unit) // 64
+        ws += (128 if method == 2 else 96) * j["num_freq"] * Np * stages
+    for j, Np in zip(jobs, nps):
+        T, F, rows = j["time_steps"], j["num_freq"], 0
+        for t0 in range(0, T, chunk):
+            stages = -(-min(chunk, T - t0) // unit) * unit // 64
+            if method == 2:
+                units = 4 * stages
+            else:
+                hpu = 1 if Np >= 1024 else 1024 // Np
+                units = -(-2 * stages // hpu)
+            per = min(max(units // 8, 1), 16)
+            rows = max(rows, -(-units // per))
+        ws += r256(rows * F * Np * 4) + r256(2 * F * Np * 8)
+    return ws
+
+
 @pytest.mark.parametrize("method", [2, 3])
 @pytest.mark.parametrize("tuning", [None, dict(max_item_stages=64), dict(max_item_stages=6),
                                     dict(workspace_bytes=1 << 30),
@@ -328,6 +354,13 @@ def test_batch_plan_host_only(tuning, method):
     budget = t.get("workspace_bytes", 16 << 30)
     maxT = max(j["time_steps"] for j in jobs)
     chunk = min(budget // per_unit * unit, -(-maxT // unit) * unit)
+    # The chunk then shrinks until planes, code-sum rows and offset
+    # corrections fit the budget together.
+    ws = _batch_workspace(jobs, chunk, unit, method)
+    while ws > budget and chunk > unit:
+        c = int(chunk * budget / ws) // unit * unit
+        chunk = max(min(c, chunk - unit), unit)
+        ws = _batch_workspace(jobs, chunk, unit, method)
     max_st = -(-t.get("max_item_stages", 0) // ring) * ring
     small = t.get("small_tile_elements", 0)
     rounds, items = _expected_items(jobs, chunk, max_st, unit, method, small)
