@@ -270,6 +270,8 @@ def main() -> int:
                     help="time one path only (for profiling)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the uneven-batch and streaming-engine figures")
+    ap.add_argument("--extra", choices=["batch_uneven64", "serve_streaming", "large_long"],
+                    help="only run this one of the extra workloads (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -287,6 +289,11 @@ def main() -> int:
         return 2
     if args.copy_calibration:
         copy_calibration(torch)
+        return 0
+    if args.extra:
+        fn = {"batch_uneven64": batch_uneven64, "serve_streaming": serve_streaming,
+              "large_long": large_long}[args.extra]
+        print(json.dumps({args.extra: fn(torch, chx)}), flush=True)
         return 0
     dev = torch.device("cuda:0")
     F, T, N = N_FREQ, T_SAMPLES, N_ELEM

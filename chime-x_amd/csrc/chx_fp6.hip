@@ -322,30 +322,6 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t id, int64_t total) {
   return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
 }
 
-// Tile tb of one channel.  First the full 128 x 256 tiles, row by row: row
-// tile ti holds floor((ntiles - ti) / 2) of them, columns from kTile ct, ct
-// = ti, ti + 2, ...  Then the half tiles, the last tile of each row tile ti
-// with ntiles - ti odd (ti = ntiles-1, ntiles-3, ...), whose 128 columns
-// are column tile ntiles-1: two of them form one item (rows ti_a for waves
-// 0-1, ti_b = ti_a - 2 for waves 2-3); an odd last one stays alone (ti_b <
-// 0; its waves 2-3 compute past the array).  Full tiles return ti_b = ti_a.
-__device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti_a, int &ti_b,
-                                            int &ct) {
-  int r = 0;
-  while (r < ntiles && tb >= (ntiles - r) / 2) {
-    tb -= (ntiles - r) / 2;
-    r++;
-  }
-  if (r < ntiles) {
-    ti_a = ti_b = r;
-    ct = r + 2 * tb;
-  } else {
-    ct = ntiles - 1;
-    ti_a = ntiles - 1 - 4 * tb;
-    ti_b = ti_a - 2;
-  }
-}
-
 // Small tile tb of one channel: row-major over the upper triangle of
 // 128 x 128 tiles (row tile ti holds ntiles - ti of them).
 __device__ __forceinline__ void tile_coords_small(int tb, int ntiles, int &ti, int &ct) {

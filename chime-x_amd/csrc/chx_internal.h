@@ -92,6 +92,53 @@ __host__ __device__ inline int64_t wide_tiles(int64_t ntiles) {
 __host__ __device__ inline int64_t paired_tiles(int64_t ntiles) {
   return wide_tiles(ntiles) - (ntiles + 1) / 2 / 2;
 }
+// Row tiles per band of the full-tile order.
+constexpr int kTileBand = 8;
+// Work item tb of one channel (both matrix-core GEMMs, 128 x 256 tiles).
+// Full tiles (rows from tile ti, columns from tiles ct and ct + 1, ct = ti,
+// ti + 2, ... <= ntiles - 2: row tile ti holds floor((ntiles - ti) / 2) of
+// them) come first, band by band: bands of kTileBand row tiles in order; in
+// a band, column tile by column tile (ct from the band's first row tile
+// on), and in a column its rows in order (the band's rows of ct's parity
+// up to ct).  The 32 items one XCD runs together then read the planes of
+// about 8 row tiles and 9 column tiles whatever the array size (row by row
+// over a channel they read every column).  Then the half tiles, the last
+// tile of each row tile ti with ntiles - ti odd (ti = ntiles-1, ntiles-3,
+// ...), whose 128 columns are column tile ntiles-1: two of them form one
+// item (rows ti_a for waves 0-1, ti_b = ti_a - 2 for waves 2-3); an odd last
+// one stays alone (ti_b < 0; its waves 2-3 compute past the array).  Full
+// tiles return ti_b = ti_a.
+__host__ __device__ inline void tile_coords(int tb, int ntiles, int &ti_a, int &ti_b,
+                                            int &ct) {
+  const int nfull = (int)wide_tiles(ntiles) - (ntiles + 1) / 2;
+  if (tb >= nfull) {
+    tb -= nfull;
+    ct = ntiles - 1;
+    ti_a = ntiles - 1 - 4 * tb;
+    ti_b = ti_a - 2;
+    return;
+  }
+  int r0 = 0;
+  for (;; r0 += kTileBand) {  // the band holding tb
+    const int r1 = r0 + kTileBand < ntiles ? r0 + kTileBand : ntiles;
+    int band = 0;
+    for (int ti = r0; ti < r1; ti++) band += (ntiles - ti) / 2;
+    if (tb < band) break;
+    tb -= band;
+  }
+  const int rl = (r0 + kTileBand < ntiles ? r0 + kTileBand : ntiles) - 1;  // last row
+  for (int c = r0;; c++) {  // column tile c holds the band's rows of c's parity up to c
+    const int a = r0 + ((c - r0) & 1);  // first such row
+    const int b = c < rl ? c : rl;
+    const int n = b >= a ? (b - a) / 2 + 1 : 0;
+    if (tb < n) {
+      ti_a = ti_b = a + 2 * tb;
+      ct = c;
+      return;
+    }
+    tb -= n;
+  }
+}
 size_t fp6_workspace_bytes(const chx_plan *p);
 int launch_corr_fp6(const chx_plan *p, const uint8_t *in, int32_t *vis, int mode,
                     hipStream_t s);

@@ -312,30 +312,6 @@ __global__ __launch_bounds__(256) void u4_corrections_batch_kernel(
   u4_corrections_block(J.partials, J.corr, J.F, J.Np, J.nrun, J.n, blk);
 }
 
-// Tile tb of one channel.  First the full 128 x 256 tiles, row by row: row
-// tile ti holds floor((ntiles - ti) / 2) of them, columns from kTile ct, ct
-// = ti, ti + 2, ...  Then the half tiles, the last tile of each row tile ti
-// with ntiles - ti odd (ti = ntiles-1, ntiles-3, ...), whose 128 columns
-// are column tile ntiles-1: two of them form one item (rows ti_a for waves
-// 0-1, ti_b = ti_a - 2 for waves 2-3); an odd last one stays alone (ti_b <
-// 0; its waves 2-3 compute past the array).  Full tiles return ti_b = ti_a.
-__device__ __forceinline__ void tile_coords(int tb, int ntiles, int &ti_a, int &ti_b,
-                                            int &ct) {
-  int r = 0;
-  while (r < ntiles && tb >= (ntiles - r) / 2) {
-    tb -= (ntiles - r) / 2;
-    r++;
-  }
-  if (r < ntiles) {
-    ti_a = ti_b = r;
-    ct = r + 2 * tb;
-  } else {
-    ct = ntiles - 1;
-    ti_a = ntiles - 1 - 4 * tb;
-    ti_b = ti_a - 2;
-  }
-}
-
 // Pass 2, one output item of channel f over 64-sample stages [st_begin,
 // st_begin + nst) (nst a multiple of kMfmaRing), run as 32-sample k-steps.
 // Wave w computes 128 rows x 64 columns as 4x2 blocks of two int32

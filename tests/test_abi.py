@@ -218,15 +218,42 @@ def _tiles(nt, method):
     return wide - (nt + 1) // 2 // 2
 
 
+def _full_tile_order(nt, band=8):
+    """(row tile, first column tile) of the full 128 x 256 tiles in the
+    kernels' item order: bands of 8 row tiles; in a band, column tile by
+    column tile from the band's first row on, and in a column the band's
+    rows of its parity up to it (ct >= ti, ct - ti even, ct <= nt - 2)."""
+    out = []
+    for r0 in range(0, nt, band):
+        for ct in range(r0, nt - 1):
+            out += [(ti, ct) for ti in range(r0, min(r0 + band, nt, ct + 1))
+                    if (ct - ti) % 2 == 0]
+    return out
+
+
+def test_full_tile_order_is_bands_of_row_tiles():
+    # Every full tile once; and any 32 consecutive items (the workgroups of
+    # one XCD) read the planes of at most 22 tiles of 128 elements however
+    # large the array (row by row: every column tile of the channel).
+    for nt in range(1, 70):
+        order = _full_tile_order(nt)
+        assert sorted(order) == [(r, r + 2 * k) for r in range(nt) for k in range((nt - r) // 2)]
+
+    def touched(win):
+        return len({t for r, c in win for t in (r, c, c + 1)})
+    for nt in (32, 64, 128):
+        order = _full_tile_order(nt)
+        assert max(touched(order[i:i + 32]) for i in range(len(order) - 31)) <= 22
+
+
 def _item_blocks(nt):
     """The 32x32 blocks each work item of one channel stores, following
-    the kernel's item order: full tiles row by row, then pairs of half tiles
-    (rows nt-1-4q and nt-3-4q, column tile nt-1), an odd last one alone."""
+    the kernel's item order: full tiles block by block, then pairs of half
+    tiles (rows nt-1-4q and nt-3-4q, column tile nt-1), an odd last one
+    alone."""
     items = []
-    for r in range(nt):
-        for k in range((nt - r) // 2):
-            ct = r + 2 * k
-            items.append({(4 * r + i, 4 * ct + j) for i in range(4) for j in range(8)})
+    for r, ct in _full_tile_order(nt):
+        items.append({(4 * r + i, 4 * ct + j) for i in range(4) for j in range(8)})
     halves = list(range(nt - 1, -1, -2))
     for q in range(0, len(halves), 2):
         rows = halves[q:q + 2]

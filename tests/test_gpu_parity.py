@@ -207,8 +207,10 @@ def test_half_tile_pairs_against_oracle(method):
     # Arrays whose row tiles end in half tiles: N=512 (4 tiles: one pair),
     # 640 (5 tiles: a pair and a lone half tile), 1152 (9 tiles: two pairs
     # and a lone one), 1376 (11 tiles, N not a multiple of 128: three
-    # pairs), checked against the oracle for both conventions.
-    for N, seed in ((512, 51), (640, 64), (1152, 11), (1376, 13)):
+    # pairs), 2944 (23 tiles: full tiles in three bands of row tiles, the
+    # last one of 7, and six pairs), checked against the oracle for both
+    # conventions.
+    for N, seed in ((512, 51), (640, 64), (1152, 11), (1376, 13), (2944, 29)):
         data = rand(256, 2, N, seed)
         for U in (1, 0):
             np.testing.assert_array_equal(run(data, method, U, 32), O.correlate_blocks(data, U),
