@@ -354,7 +354,15 @@ __device__ __forceinline__ void tile_coords_small(int tb, int ntiles, int &ti, i
 // every partial sum is an integer of magnitude <= 2^24, exact), and the
 // flush adds 2^24 and the offset corrections J.corr (the piece that starts
 // the chunk adds them, once).
-template <bool PAIR, bool SMALL = false>
+// SKIP: bit 2i + j set = the wave's block (row group i, column group j) is
+// never stored (below the diagonal of a diagonal tile), so its MFMAs are
+// left out, and a row group with both blocks left out reads no row
+// operands.  Every column group must keep an MFMA: a column load that no
+// MFMA uses would be deleted, and the vmcnt waits below count on every
+// wave issuing the same loads per stage.  Loads, DMAs and barriers are
+// then the same in every variant, so the waves of one workgroup may run
+// different ones.
+template <bool PAIR, bool SMALL = false, int SKIP = 0>
 __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, const int ti_a,
                                              const int ti_b, const int ct, const int st_begin,
                                              const int nst, uint8_t (*lds)[kSlotBytes]) {
@@ -387,6 +395,11 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   static_assert(kDma == kP0n + kP1n, "the waits below count kDma + 8 loads per stage");
   static_assert(!(PAIR && SMALL), "a small tile is one 128-row block");
   constexpr int kG = SMALL ? 2 : 4;                  // row groups this wave computes
+  auto skipped = [](int i, int j) constexpr { return ((SKIP >> (2 * i + j)) & 1) != 0; };
+  auto row_used = [&](int i) constexpr { return !skipped(i, 0) || !skipped(i, 1); };
+  static_assert((SKIP & 0x55 & (SMALL ? 0x05 : 0x55)) != (SMALL ? 0x05 : 0x55) &&
+                    (SKIP & 0xAA & (SMALL ? 0x0A : 0xAA)) != (SMALL ? 0x0A : 0xAA),
+                "every column group keeps an MFMA (its loads stay in the counted waits)");
   const int gr0 = SMALL ? 2 * (wave >> 1) : 0;       // the first of them
   const int rb = PAIR ? wave >> 1 : 0;               // this wave's row block
   const int lp = PAIR ? wave & 1 : wave >> 1;        // plane it fills
@@ -500,9 +513,11 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
     const v8i nbr[2] = {br0 ^ sign, br1 ^ sign};
 #pragma unroll
     for (int i = 0; i < kG; i++) {
+      if (!row_used(i)) continue;
       const v8i xr = AR[p][i], xi = AI[p][i];
 #pragma unroll
       for (int j = 0; j < 2; j++) {
+        if (skipped(i, j)) continue;
         re[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
             xr, br[j], re[i][j], kFmtE2M3, kFmtE2M3, 0, kScale2p3, 0, kScale2p3);
         im[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
@@ -510,6 +525,7 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
       }
 #pragma unroll
       for (int j = 0; j < 2; j++) {
+        if (skipped(i, j)) continue;
         re[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
             xi, bi[j], re[i][j], kFmtE2M3, kFmtE2M3, 0, kScale2p3, 0, kScale2p3);
         im[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
@@ -548,6 +564,7 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
     asm volatile("s_waitcnt vmcnt(33)\n\ts_barrier" ::: "memory");
 #pragma unroll
   for (int i = 0; i < kG; i++) {
+    if (!row_used(i)) continue;
     AR[0][i] = load_op(0, 0, gr0 + i);
     AI[0][i] = load_op(0, 1, gr0 + i);
   }
@@ -578,8 +595,10 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
       for (int g = 0; g < kG; g++)
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-          if (i % 2 == 1) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+          // (a row group's MFMAs: 4 per block not left out)
+          if (i < 4 * (!skipped(g, 0) + !skipped(g, 1)))
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+          if (i % 2 == 1 && row_used(g)) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
           if (SMALL) {
             if (i < (g == 0 ? 6 : 5)) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
           } else if (g < 3 && i < (PAIR ? (g == 2 ? 4 : 5) : (g == 2 ? 3 : 4))) {
@@ -628,6 +647,27 @@ __device__ __forceinline__ void gemm_tile_f6(const GemmJob &J, const int f, cons
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
+// A tile with each wave on its own variant: the waves whose blocks include
+// some below the diagonal (tile_skip_mask) leave out their MFMAs.  Masks
+// that would leave a column group without MFMAs (a wave past the array, or
+// wholly below the diagonal of a 128 x 128 tile) run the full variant.
+template <bool PAIR, bool SMALL = false>
+__device__ __forceinline__ void gemm_tile_f6_masked(const GemmJob &J, const int f,
+                                                    const int ti_a, const int ti_b,
+                                                    const int ct, const int st_begin,
+                                                    const int nst,
+                                                    uint8_t (*lds)[kSlotBytes]) {
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int m = tile_skip_mask(PAIR, SMALL, ti_a, ti_b, ct, J.Np, wave);
+  if constexpr (SMALL) {
+    if (m == 0x04) return gemm_tile_f6<false, true, 0x04>(J, f, ti_a, ti_b, ct, st_begin, nst, lds);
+  } else {
+    if (m == 0xF4) return gemm_tile_f6<PAIR, false, 0xF4>(J, f, ti_a, ti_b, ct, st_begin, nst, lds);
+    if (m == 0x40) return gemm_tile_f6<PAIR, false, 0x40>(J, f, ti_a, ti_b, ct, st_begin, nst, lds);
+  }
+  gemm_tile_f6<PAIR, SMALL, 0>(J, f, ti_a, ti_b, ct, st_begin, nst, lds);
+}
+
 // One plan chunk: items = channels x tiles.  Workgroups 0..head-1 take
 // items 0..head-1 whole (all KB stages, mode J.mode); the rest take the
 // tail items head.. cut into time pieces of J.stages_per_split stages (the
@@ -655,9 +695,9 @@ __global__ __launch_bounds__(256, 1) void corr_f6_kernel(GemmJob J, int head, in
   int ti_a, ti_b, ct;
   tile_coords((int)(item % per_f), J.ntiles, ti_a, ti_b, ct);
   if (ti_b >= 0 && ti_b != ti_a)
-    gemm_tile_f6<true>(J, (int)(item / per_f), ti_a, ti_b, ct, s0, ns, lds);
+    gemm_tile_f6_masked<true>(J, (int)(item / per_f), ti_a, ti_b, ct, s0, ns, lds);
   else
-    gemm_tile_f6<false>(J, (int)(item / per_f), ti_a, ti_a, ct, s0, ns, lds);
+    gemm_tile_f6_masked<false>(J, (int)(item / per_f), ti_a, ti_a, ct, s0, ns, lds);
 }
 
 // Batch: persistent workgroups take work items from 8 per-XCD queues (the
@@ -695,14 +735,14 @@ __global__ __launch_bounds__(256, 1) void corr_f6_batch_kernel(
     int ti_a, ti_b, ct;
     if (J.small) {
       tile_coords_small(w.tile, J.ntiles, ti_a, ct);
-      gemm_tile_f6<false, true>(J, w.f, ti_a, ti_a, ct, w.stage0, w.nstages, lds);
+      gemm_tile_f6_masked<false, true>(J, w.f, ti_a, ti_a, ct, w.stage0, w.nstages, lds);
       continue;
     }
     tile_coords(w.tile, J.ntiles, ti_a, ti_b, ct);
     if (ti_b >= 0 && ti_b != ti_a)
-      gemm_tile_f6<true>(J, w.f, ti_a, ti_b, ct, w.stage0, w.nstages, lds);
+      gemm_tile_f6_masked<true>(J, w.f, ti_a, ti_b, ct, w.stage0, w.nstages, lds);
     else
-      gemm_tile_f6<false>(J, w.f, ti_a, ti_a, ct, w.stage0, w.nstages, lds);
+      gemm_tile_f6_masked<false>(J, w.f, ti_a, ti_a, ct, w.stage0, w.nstages, lds);
   }
 }
 

@@ -156,6 +156,27 @@ inline int format_ring(int fmt) { return fmt == kPlanesFp6 ? kFp6Ring : kMfmaRin
 inline int64_t format_tiles(int, int64_t ntiles) { return paired_tiles(ntiles); }
 // work items of one channel on 128 x 128 tiles: the upper triangle of tiles
 __host__ __device__ inline int64_t small_tiles(int64_t ntiles) { return ntiles * (ntiles + 1) / 2; }
+// The blocks of wave `wave` that no flush stores, bit 2i + j for its row
+// group i and column group j (32-element blocks): those below the diagonal
+// of a diagonal tile, and every block of a wave whose columns lie past the
+// array.  Wave layouts as the GEMM tiles assign them: a 128 x 256 tile
+// (rows ti_a, columns from kMfmaTile ct, wave w: all 4 row groups x column
+// groups 2w, 2w+1); a pair of half tiles (waves 0-1 rows ti_a, 2-3 rows
+// ti_b, columns from kMfmaTile ct + 64 (w%2)); a 128 x 128 tile (wave w:
+// row groups 2(w/2) + i, i < 2, column groups 2(w%2) + j).
+__host__ __device__ inline int tile_skip_mask(bool pair, bool small, int ti_a, int ti_b, int ct,
+                                              int Np, int wave) {
+  if (!pair && !small && wave >= 2 && (ct + 1) * kMfmaTile >= Np) return 0xFF;
+  const int ti = pair && wave >= 2 ? ti_b : ti_a;
+  const int gr0 = small ? 2 * (wave >> 1) : 0;
+  const int cw = pair || small ? wave & 1 : wave;
+  const int ng = small ? 2 : 4;
+  int m = 0;
+  for (int i = 0; i < ng; i++)
+    for (int j = 0; j < 2; j++)
+      if (4 * ct + 2 * cw + j < 4 * ti + gr0 + i) m |= 1 << (2 * i + j);
+  return m;
+}
 // the most stages one work item may cover: FP6 work items hold offset-binary
 // code products, at most 450 per sample, from -2^24: 1024 stages keep every
 // partial sum within 2^24 (int8: 2048)

@@ -327,7 +327,15 @@ __global__ __launch_bounds__(256) void u4_corrections_batch_kernel(
 // buffer->LDS DMA; each wave loads its own column operands straight into
 // VGPRs, three k-steps ahead, so every operand byte crosses the CU's load
 // path once.
-template <bool PAIR>
+// SKIP: bit 2i + j set = the wave's block (row group i, column group j) is
+// never stored (below the diagonal of a diagonal tile), so its MFMAs are
+// left out, and so are the row operand reads of a row group with both
+// blocks left out.  Every column group must keep an MFMA: a column load
+// that no MFMA uses would be deleted, and the vmcnt waits below count on
+// every wave issuing the same loads per k-step.  Loads, DMAs and barriers
+// are then the same in every variant, so the waves of one workgroup may
+// run different ones.
+template <bool PAIR, int SKIP = 0>
 __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const int ti_a,
                                           const int ti_b, const int ct, const int st_begin,
                                           const int nst, uint8_t (*lds)[kSlotBytes]) {
@@ -337,6 +345,9 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int k0 = 2 * st_begin, nk = 2 * nst;  // k-steps
+  auto skipped = [](int i, int j) constexpr { return ((SKIP >> (2 * i + j)) & 1) != 0; };
+  static_assert((SKIP & 0x55) != 0x55 && (SKIP & 0xAA) != 0xAA,
+                "every column group keeps an MFMA (its loads stay in the counted waits)");
 
   // An operand (32 elements x 32 samples) is 1 KB: lanes 0-31 the k-step's
   // even 16-sample block, 32-63 the odd one.
@@ -419,14 +430,17 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
       const v4i xr = i == 3 ? ar3n : ar[i], xi = i == 3 ? ai3n : ai[i];
 #pragma unroll
       for (int j = 0; j < 2; j++) {
+        if (skipped(i, j)) continue;
         re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(xr, br[j], re[i][j], 0, 0, 0);
         im[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(xr, bi[j], im[i][j], 0, 0, 0);
       }
 #pragma unroll
       for (int j = 0; j < 2; j++) {
+        if (skipped(i, j)) continue;
         re[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(xi, bi[j], re[i][j], 0, 0, 0);
         im[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(xi, nbr[j], im[i][j], 0, 0, 0);
       }
+      // (the reads of a row group no MFMA uses are dead and left out)
       if (i < 3) {
         ar[i] = load_op(nslot, 0, i);
         ai[i] = load_op(nslot, 1, i);
@@ -493,28 +507,34 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
       issue_rows(cl(ks + 4), d);
       load_cols(BR[n3][0], BR[n3][1], BI[n3][0], BI[n3][1], cl(ks + 3));
       step_ops(BR[d][0], BR[d][1], BI[d][0], BI[d][1], (d + 1) % kRing);
-      // 8 MFMAs per row group i; VMEM ops (kDma DMAs, 4 column loads) over
-      // groups 0-1, LDS reads of the next row operands from group 1 on
+      // 8 MFMAs per row group i (4 per block not left out); VMEM ops (kDma
+      // DMAs, 4 column loads) over groups 0-1, LDS reads of the next row
+      // operands from group 1 on (group 1: row group 0's, 2: 1's and 3's,
+      // 3: 2's)
+      constexpr int m0 = 4 * (!skipped(0, 0) + !skipped(0, 1));
+      constexpr int m1 = 4 * (!skipped(1, 0) + !skipped(1, 1));
+      constexpr int m2 = 4 * (!skipped(2, 0) + !skipped(2, 1));
+      constexpr int m3 = 4 * (!skipped(3, 0) + !skipped(3, 1));
 #pragma unroll
       for (int i = 0; i < 8; i++) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (i < m0) __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
         if (i < (PAIR ? 4 : 3)) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
       }
 #pragma unroll
       for (int i = 0; i < 8; i++) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-        if (i < 2) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        if (i < m1) __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (i < (m0 ? 2 : 0)) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
         if (i >= 4 && i < (PAIR ? 8 : 7)) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
       }
 #pragma unroll
       for (int i = 0; i < 8; i++) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-        if (i < 4) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        if (i < m2) __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (i < (m1 ? 2 : 0) + (m3 ? 2 : 0)) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
       }
 #pragma unroll
       for (int i = 0; i < 8; i++) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-        if (i < 2) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        if (i < m3) __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        if (i < (m2 ? 2 : 0)) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
       }
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -560,6 +580,22 @@ __device__ __forceinline__ void gemm_tile(const GemmJob &J, const int f, const i
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
+// A tile with each wave on its own variant: the waves whose blocks include
+// some below the diagonal (tile_skip_mask) leave out their MFMAs.  A wave
+// past the array (mask 0xFF) runs the full variant: its column loads must
+// stay.
+template <bool PAIR>
+__device__ __forceinline__ void gemm_tile_masked(const GemmJob &J, const int f, const int ti_a,
+                                                 const int ti_b, const int ct,
+                                                 const int st_begin, const int nst,
+                                                 uint8_t (*lds)[kSlotBytes]) {
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int m = tile_skip_mask(PAIR, false, ti_a, ti_b, ct, J.Np, wave);
+  if (m == 0xF4) return gemm_tile<PAIR, 0xF4>(J, f, ti_a, ti_b, ct, st_begin, nst, lds);
+  if (m == 0x40) return gemm_tile<PAIR, 0x40>(J, f, ti_a, ti_b, ct, st_begin, nst, lds);
+  gemm_tile<PAIR, 0>(J, f, ti_a, ti_b, ct, st_begin, nst, lds);
+}
+
 // One plan chunk: items = channels x tiles.  Workgroups 0..head-1 take
 // items 0..head-1 whole (all stages, mode J.mode); the rest take the tail
 // items head.. cut into time pieces of J.stages_per_split stages (the last
@@ -585,9 +621,9 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_kernel(GemmJob J, int nst, i
   int ti_a, ti_b, ct;
   tile_coords((int)(item % per_f), J.ntiles, ti_a, ti_b, ct);
   if (ti_b >= 0 && ti_b != ti_a)
-    gemm_tile<true>(J, (int)(item / per_f), ti_a, ti_b, ct, s0, ns, lds);
+    gemm_tile_masked<true>(J, (int)(item / per_f), ti_a, ti_b, ct, s0, ns, lds);
   else
-    gemm_tile<false>(J, (int)(item / per_f), ti_a, ti_a, ct, s0, ns, lds);
+    gemm_tile_masked<false>(J, (int)(item / per_f), ti_a, ti_a, ct, s0, ns, lds);
 }
 
 // Batch: persistent workgroups take work items from 8 per-XCD queues: a
@@ -625,9 +661,9 @@ __global__ __launch_bounds__(256, 1) void corr_mfma_batch_kernel(
     int ti_a, ti_b, ct;
     tile_coords(w.tile, J.ntiles, ti_a, ti_b, ct);
     if (ti_b >= 0 && ti_b != ti_a)
-      gemm_tile<true>(J, w.f, ti_a, ti_b, ct, w.stage0, w.nstages, lds);
+      gemm_tile_masked<true>(J, w.f, ti_a, ti_b, ct, w.stage0, w.nstages, lds);
     else
-      gemm_tile<false>(J, w.f, ti_a, ti_a, ct, w.stage0, w.nstages, lds);
+      gemm_tile_masked<false>(J, w.f, ti_a, ti_a, ct, w.stage0, w.nstages, lds);
   }
 }
 
