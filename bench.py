@@ -134,10 +134,12 @@ def cpu_baseline(budget_s: float = 12.0):
 
 def from_profile(kernel: str, key: str):
     """A figure recorded in the kernel's rocprofv3 summary under profiles/:
-    traffic_bytes_per_launch (HBM bytes per GEMM launch from PMC FETCH_SIZE +
-    WRITE_SIZE, each scaled by the same counters' reading for a copy of known
-    size) or mfma_busy_pct (MfmaUtil: SQ_VALU_MFMA_BUSY_CYCLES over
-    GRBM_GUI_ACTIVE x SIMDs, steady launches)."""
+    traffic_bytes_per_launch (bytes per GEMM launch between the L2s and
+    memory: the L2's 128-byte read requests and 64-byte write requests to
+    the fabric, PMC, checked against a device copy of known size),
+    mfma_busy_pct (MfmaUtil: SQ_VALU_MFMA_BUSY_CYCLES over GRBM_GUI_ACTIVE x
+    SIMDs, steady launches) or clock_ghz (GRBM_GUI_ACTIVE per XCD over the
+    kernel time)."""
     import re
     path = os.path.join(HERE, "profiles", f"{kernel.replace('_kernel', '')}_N2048_F16_T65536.md")
     try:
