@@ -89,6 +89,14 @@ extern "C" int chx_correlate_noalloc(const chx_config *cfg, const uint8_t *d_inp
   int cur = -1;
   CHX_HIP(hipGetDevice(&cur));
   if (cur != cfg->device) CHX_HIP(hipSetDevice(cfg->device));
+  // the caller's current device is restored on every return below
+  struct Restore {
+    int dev;
+    bool on;
+    ~Restore() {
+      if (on) (void)hipSetDevice(dev);
+    }
+  } restore{cur, cur != cfg->device};
   uint32_t *base = nullptr;
   rc = scratch_ptr(cfg->device, &base);
   if (rc) return rc;
