@@ -2,6 +2,7 @@
 header declares, its host helpers match the oracle / reference fixtures, and
 bad shapes are refused before anything touches a device."""
 import ctypes
+import os
 import subprocess
 
 import numpy as np
@@ -508,3 +509,84 @@ def test_feed16_to_triangle_matches_reference(golden):
     assert L.chx_feed16_to_triangle(1, 1, full16.ctypes.data, None) == chx.ERR_INVALID_ARGUMENT
     assert L.chx_feed16_to_triangle(-1, 1, full16.ctypes.data,
                                     full16.ctypes.data) == chx.ERR_INVALID_ARGUMENT
+
+
+_SKIP_PROBE = r"""
+#include "chx_internal.h"
+#include <cstdio>
+#include <set>
+// Every item of one channel, every wave: the blocks tile_skip_mask leaves
+// out must be blocks the flush never stores (bx < by, or bx >= nb), with
+// the wave layouts the GEMM tiles use (by0 = 4 ti + gr0, bx0 = 4 ct + 2 cw).
+int main() {
+  std::set<int> masks[2];
+  for (int nt = 1; nt <= 40; nt++) {
+    const int nb = 4 * nt, Np = 128 * nt;
+    for (int small = 0; small < 2; small++) {
+      const int items = small ? (int)chx::small_tiles(nt) : (int)chx::paired_tiles(nt);
+      for (int tb = 0; tb < items; tb++) {
+        int ti_a, ti_b, ct;
+        if (small) {
+          int r = 0, t = tb;
+          while (r < nt - 1 && t >= nt - r) { t -= nt - r; r++; }
+          ti_a = ti_b = r;
+          ct = r + t;
+        } else {
+          chx::tile_coords(tb, nt, ti_a, ti_b, ct);
+        }
+        const bool pair = !small && ti_b >= 0 && ti_b != ti_a;
+        for (int w = 0; w < 4; w++) {
+          const int m = chx::tile_skip_mask(pair, small, ti_a, ti_b, ct, Np, w);
+          masks[small].insert(m);
+          const int ti = pair && w >= 2 ? ti_b : ti_a;
+          const int gr0 = small ? 2 * (w >> 1) : 0;
+          const int cw = pair || small ? w & 1 : w;
+          const int ng = small ? 2 : 4;
+          for (int i = 0; i < ng; i++)
+            for (int j = 0; j < 2; j++) {
+              const int by = 4 * ti + gr0 + i, bx = 4 * ct + 2 * cw + j;
+              const bool stored = bx >= by && bx < nb;
+              if (((m >> (2 * i + j)) & 1) && stored) {
+                printf("BAD nt=%d small=%d tb=%d w=%d block=(%d,%d) mask=%x\n", nt, small, tb, w,
+                       by, bx, m);
+                return 1;
+              }
+            }
+          if (m >> (small ? 4 : 8)) { printf("BAD mask %x\n", m); return 1; }
+        }
+      }
+    }
+  }
+  for (int s = 0; s < 2; s++) {
+    printf("masks%d", s);
+    for (int m : masks[s]) printf(" %x", m);
+    printf("\n");
+  }
+  return 0;
+}
+"""
+
+
+def test_tile_skip_masks_leave_out_only_unstored_blocks(tmp_path):
+    # The GEMMs' per-wave skip masks (chx_internal.h, compiled here for the
+    # host) never leave out a block that a flush stores, over every item of
+    # arrays of 1-40 tiles of 128 elements; and the masks that occur are the
+    # ones the kernels have variants for (0xF4, 0x40 and, for a wave past the
+    # array, 0xFF; 128 x 128 tiles: 0x04 and 0x0F) or no skip at all.
+    import shutil
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no host C++ compiler")
+    src = tmp_path / "skip_probe.cpp"
+    src.write_text(_SKIP_PROBE)
+    exe = tmp_path / "skip_probe"
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run([gxx, "-std=c++17", "-O1", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                    "-I" + os.path.join(root, "include"),
+                    "-I" + os.path.join(root, "chime-x_amd", "csrc"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout
+    lines = dict(l.split(" ", 1) for l in out.stdout.strip().splitlines())
+    assert set(lines["masks0"].split()) == {"0", "f4", "40", "ff"}
+    assert set(lines["masks1"].split()) == {"0", "4", "f"}
