@@ -133,7 +133,9 @@ int chx_correlate_host(chx_plan *plan, const uint8_t *h_input, int32_t *h_vis);
  * process ends; a larger request returns CHX_ERR_BUDGET (chx_last_error_detail
  * says how much it needs).  Each call rewrites the scratch before use, but
  * all calls on one GPU share it: calls that overlap on one GPU (other
- * streams, other threads) race.  Assigns d_vis like chx_correlate. */
+ * streams, other threads) race.  Assigns d_vis like chx_correlate.  Works
+ * on cfg->device, whichever device is current, and leaves the calling
+ * thread's current device as it found it. */
 #define CHX_NOALLOC_SCRATCH_BYTES (2u << 20)
 int chx_correlate_noalloc(const chx_config *cfg, const uint8_t *d_input, int32_t *d_vis,
                           void *stream);
