@@ -34,12 +34,16 @@
 //     waves, come through a ring of 4 LDS stages of one 64-sample k-step
 //     (12 KB each) filled by buffer->LDS DMA; each wave loads its own
 //     column operands straight into VGPRs, three stages ahead, so every
-//     operand byte crosses the CU's load path once.  The load path bounds
-//     the kernel: each load instruction holds up the wave's in-order issue,
-//     and a DMA costs about 1.5x a direct load of the same bytes.  The half
-//     tiles at the array's right edge go two to a workgroup (one 128-row
-//     block per pair of waves, each with its own ring).  The batch kernel
-//     can also run 128 x 128 tiles (each wave 64 x 64) for small arrays.
+//     operand byte crosses the CU's load path once.  The power-limited
+//     clock and the load path bound the kernel: each load instruction holds
+//     up the wave's in-order issue, and a DMA costs about 1.5x a direct load
+//     of the same bytes.  The half tiles at the array's right edge go two
+//     to a workgroup (one 128-row block per pair of waves, each with its
+//     own ring).  On a tile that straddles the diagonal, the waves whose
+//     blocks lie partly below it run a k-loop variant without those
+//     blocks' MFMAs (less power drawn, so a higher clock for every CU).
+//     The batch kernel can also run 128 x 128 tiles (each wave 64 x 64) for
+//     small arrays.
 #include "chx_internal.h"
 
 namespace chx {

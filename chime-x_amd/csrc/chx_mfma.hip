@@ -33,7 +33,9 @@
 //     x 64 columns (4x2 blocks x 2 int32 accumulators: RE' and IM').  The
 //     row operands come through a ring of 4 LDS k-steps of 32 samples (8 KB
 //     each) filled by buffer->LDS DMA; each wave loads its own column
-//     operands straight into VGPRs, three k-steps ahead.
+//     operands straight into VGPRs, three k-steps ahead.  On a tile that
+//     straddles the diagonal, the waves whose blocks lie partly below it
+//     run a k-loop variant without those blocks' MFMAs.
 // A work item covers at most kMaxItemStages 64-sample stages (131072
 // samples): code products add at most 450 per sample, < 2^26 per
 // accumulator.
